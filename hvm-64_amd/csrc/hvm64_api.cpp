@@ -1,0 +1,486 @@
+// Engine object and the C ABI (include/hvm64_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/hvm64_amd.h"
+#include "hvm64_engine.h"
+#include "hvm64_host.h"
+
+namespace hvm {
+hipError_t launch_fill(u64 *words, u64 n, u64 value, hipStream_t st);
+hipError_t launch_exec(const Params &P, u32 def, u64 t0, u32 count_dref, hipStream_t st);
+hipError_t launch_expand(const Params &P, u64 old_root_cell, hipStream_t st);
+hipError_t launch_reduce(const Params &P, u32 blocks, hipStream_t st);
+hipError_t launch_readback(const Params &P, u64 root_cell, u64 *stack, u64 stack_cap, u64 *out, u64 out_cap,
+                           hipStream_t st);
+u32 reduce_waves_per_block();
+hipError_t reduce_occupancy(int *blocks_per_cu);
+}  // namespace hvm
+
+using namespace hvm;
+using hvm64::Error;
+
+struct hvm_gpu {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  u64 *heap = nullptr;
+  u64 heap_words = 0;
+  Globals *g = nullptr;
+  u64 *queue = nullptr;
+  u64 qslots = 0;
+  DefRec *d_defs = nullptr;
+  u64 *d_instrs = nullptr;
+  u64 *d_labbits = nullptr;
+  u32 ndefs = 0;
+  u64 *d_trg = nullptr;
+  u32 trg_slots = 0;
+  u32 blocks = 0, waves = 0;
+  u64 *d_rb_stack = nullptr, *d_rb_out = nullptr;
+  u64 rb_cap = 0;
+  hvm64::Host host;
+  std::vector<DefRec> defs_host;
+  std::vector<u64> instrs_host, labbits_host;
+  u64 root_cell = 0;
+  Globals last{};
+  double last_ms = 0;
+  u64 timeout_ticks = 0;  // wall-clock ticks a reduction may run (0 = unbounded)
+  u64 wall_khz = 100000;
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string g_open_error;
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw Error(HVM_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+Params make_params(hvm_gpu *h, u64 limit) {
+  Params P{};
+  P.heap = h->heap;
+  P.heap_words = h->heap_words;
+  u64 usable = h->heap_words - RESERVED_WORDS;
+  u64 lane_words = usable / 8 * 7;
+  u64 lanes = (u64)h->waves * 64;
+  P.lane_node0 = RESERVED_WORDS / 2;
+  P.nodes_per_lane = lane_words / 2 / lanes;
+  P.ov_start = RESERVED_WORDS + P.nodes_per_lane * lanes * 2;
+  P.ov_end = h->heap_words;
+  P.defs = h->d_defs;
+  P.instrs = h->d_instrs;
+  P.labbits = h->d_labbits;
+  P.trg_buf = h->d_trg;
+  P.queue = h->queue;
+  P.qmask = h->qslots - 1;
+  P.g = h->g;
+  P.ndefs = h->ndefs;
+  P.trg_slots = h->trg_slots;
+  P.total_waves = h->waves;
+  P.limit = limit;
+  P.timeout_ticks = h->timeout_ticks;
+  return P;
+}
+
+void sync_globals(hvm_gpu *h) {
+  hip_check(hipStreamSynchronize(h->stream), "kernel");
+  hip_check(hipMemcpy(&h->last, h->g, sizeof(Globals), hipMemcpyDeviceToHost), "read globals");
+  if (h->last.err) {
+    u32 e = h->last.err;
+    int code = e == ERR_OOM ? HVM_E_OOM
+               : e == ERR_UNIMPL ? HVM_E_UNIMPL_PAIR
+               : e == ERR_DIV0 ? HVM_E_DIV0
+               : e == ERR_QUEUE ? HVM_E_QUEUE
+               : e == ERR_READBACK ? HVM_E_READBACK
+               : e == ERR_TIMEOUT ? HVM_E_TIMEOUT
+                                   : HVM_E_DEVICE;
+    const char *msg = e == ERR_OOM ? "OOM"
+                      : e == ERR_UNIMPL ? "unimplemented interaction (Switch-Ctr(0) or Op-Switch)"
+                      : e == ERR_DIV0 ? "integer division by zero"
+                      : e == ERR_QUEUE ? "global redex ring overflow"
+                      : e == ERR_READBACK ? "readback failed"
+                      : e == ERR_TIMEOUT ? "reduction exceeded its time limit"
+                      : e == ERR_BADPTR ? "invalid heap or book address (engine bug)"
+                                          : "device error";
+    throw Error(code, msg);
+  }
+}
+
+// (Re)size the per-wave trg register file to the book's widest definition.
+void ensure_trg(hvm_gpu *h, u32 slots) {
+  if (slots <= h->trg_slots && h->d_trg) return;
+  u32 s = std::max<u32>(slots, 16);
+  dfree(h->d_trg);
+  hip_check(hipMalloc(&h->d_trg, (size_t)h->waves * s * 64 * sizeof(u64)), "hipMalloc trg");
+  h->trg_slots = s;
+}
+
+void upload_book(hvm_gpu *h) {
+  dfree(h->d_defs);
+  dfree(h->d_instrs);
+  dfree(h->d_labbits);
+  h->ndefs = (u32)h->defs_host.size();
+  auto up = [&](auto *&dst, const auto &vec) {
+    size_t bytes = std::max<size_t>(vec.size() * sizeof(vec[0]), 16);
+    hip_check(hipMalloc(&dst, bytes), "hipMalloc book");
+    if (!vec.empty()) hip_check(hipMemcpy(dst, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice), "upload book");
+  };
+  up(h->d_defs, h->defs_host);
+  up(h->d_instrs, h->instrs_host);
+  up(h->d_labbits, h->labbits_host);
+  u32 slots = 1;
+  for (auto &d : h->defs_host) slots = std::max(slots, d.nslots);
+  ensure_trg(h, slots);
+}
+
+void reset_net(hvm_gpu *h) {
+  hip_check(launch_fill(h->heap, h->heap_words, P_FREE, h->stream), "fill heap");
+  hip_check(launch_fill(h->queue, h->qslots * 2, 0, h->stream), "clear ring");
+  Globals g0{};
+  Params P = make_params(h, ~0ull);
+  g0.ov_top = P.ov_start;
+  hip_check(hipMemcpyAsync(h->g, &g0, sizeof(Globals), hipMemcpyHostToDevice, h->stream), "init globals");
+  u64 none = P_NONE;
+  hip_check(hipMemcpyAsync(h->heap + ROOT0_WORD, &none, sizeof(u64), hipMemcpyHostToDevice, h->stream), "root");
+  h->root_cell = HVM_HEAP_OFF + ROOT0_WORD * 8;
+  hip_check(hipStreamSynchronize(h->stream), "reset");
+  h->last = g0;
+}
+
+// Run the persistent reduction kernel with a redex budget.
+void run_reduce(hvm_gpu *h, u64 limit) {
+  Params P = make_params(h, limit);
+  u64 zero = 0;
+  hip_check(hipMemcpyAsync(&h->g->done, &zero, sizeof(u64), hipMemcpyHostToDevice, h->stream), "budget");
+  hip_check(hipEventRecord(h->ev0, h->stream), "event");
+  hip_check(launch_reduce(P, h->blocks, h->stream), "launch reduce");
+  hip_check(hipEventRecord(h->ev1, h->stream), "event");
+  sync_globals(h);
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "elapsed");
+  h->last_ms = ms;
+}
+
+std::string readback(hvm_gpu *h) {
+  for (;;) {
+    if (!h->d_rb_out) {
+      h->rb_cap = std::max<u64>(h->rb_cap, 1 << 20);
+      hip_check(hipMalloc(&h->d_rb_out, h->rb_cap * sizeof(u64)), "hipMalloc readback");
+      hip_check(hipMalloc(&h->d_rb_stack, h->rb_cap * sizeof(u64)), "hipMalloc readback");
+    }
+    Params P = make_params(h, ~0ull);
+    hip_check(launch_readback(P, h->root_cell, h->d_rb_stack, h->rb_cap, h->d_rb_out, h->rb_cap, h->stream),
+              "launch readback");
+    hip_check(hipStreamSynchronize(h->stream), "readback");
+    Globals g;
+    hip_check(hipMemcpy(&g, h->g, sizeof(Globals), hipMemcpyDeviceToHost), "read globals");
+    if (g.err == ERR_READBACK && g.err_info == 1 && h->rb_cap < (1ull << 31)) {
+      u32 zero = 0;
+      hip_check(hipMemcpy(&h->g->err, &zero, sizeof(u32), hipMemcpyHostToDevice), "clear err");
+      dfree(h->d_rb_out);
+      dfree(h->d_rb_stack);
+      h->rb_cap *= 4;
+      continue;
+    }
+    if (g.err) throw Error(HVM_E_READBACK, "readback: malformed net");
+    std::vector<u64> stream(g.result);
+    if (g.result)
+      hip_check(hipMemcpy(stream.data(), h->d_rb_out, g.result * sizeof(u64), hipMemcpyDeviceToHost), "copy readback");
+    std::string text;
+    hvm64::with_big_stack([&] {
+      size_t pos = 0;
+      std::map<u64, u64> vars;
+      u64 next = 0;
+      text = hvm64::readback_text(h->host, stream, pos, vars, next);
+    });
+    return text;
+  }
+}
+
+template <class F>
+int guarded(hvm_gpu *h, F fn) {
+  if (!h) return HVM_E_ARG;
+  try {
+    fn();
+    return 0;
+  } catch (Error &e) {
+    h->err = e.what();
+    return e.code;
+  } catch (std::exception &e) {
+    h->err = e.what();
+    return HVM_E_DEVICE;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int hvm_gpu_open(int device, uint64_t heap_bytes, hvm_gpu **out) {
+  if (!out) return HVM_E_ARG;
+  *out = nullptr;
+  auto h = std::make_unique<hvm_gpu>();
+  try {
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) throw Error(HVM_E_ARG, "no such device");
+    h->device = device;
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream");
+    hip_check(hipEventCreate(&h->ev0), "event");
+    hip_check(hipEventCreate(&h->ev1), "event");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, device), "props");
+    int per_cu = 0;
+    hip_check(reduce_occupancy(&per_cu), "occupancy");
+    if (per_cu < 1) per_cu = 1;
+    h->blocks = (u32)(prop.multiProcessorCount * per_cu);
+    h->waves = h->blocks * reduce_waves_per_block();
+    u64 words = heap_bytes / 8;
+    u64 min_words = RESERVED_WORDS + (u64)h->waves * 64 * 2 * 8 + 4096;
+    if (words < min_words) words = min_words;
+    h->heap_words = words;
+    hip_check(hipMalloc(&h->heap, words * 8), "hipMalloc heap");
+    hip_check(hipMalloc(&h->g, sizeof(Globals)), "hipMalloc globals");
+    h->qslots = 1ull << 22;
+    while (h->qslots < (u64)h->waves * 512) h->qslots <<= 1;
+    hip_check(hipMalloc(&h->queue, h->qslots * 2 * sizeof(u64)), "hipMalloc ring");
+    int wall_khz = 0;
+    hip_check(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device), "wall clock rate");
+    h->wall_khz = wall_khz > 0 ? (u64)wall_khz : 100000;
+    h->timeout_ticks = h->wall_khz * 1000 * 600;  // 10 minutes unless the caller sets one
+    upload_book(h.get());
+    reset_net(h.get());
+  } catch (Error &e) {
+    g_open_error = e.what();
+    hvm_gpu_close(h.release());
+    return e.code;
+  }
+  *out = h.release();
+  return 0;
+}
+
+void hvm_gpu_close(hvm_gpu *h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  dfree(h->heap);
+  dfree(h->g);
+  dfree(h->queue);
+  dfree(h->d_defs);
+  dfree(h->d_instrs);
+  dfree(h->d_labbits);
+  dfree(h->d_trg);
+  dfree(h->d_rb_out);
+  dfree(h->d_rb_stack);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+const char *hvm_gpu_last_error(hvm_gpu *h) { return h ? h->err.c_str() : g_open_error.c_str(); }
+
+int hvm_gpu_load_book_text(hvm_gpu *h, const char *text) {
+  return guarded(h, [&] {
+    if (!text) throw Error(HVM_E_ARG, "null book");
+    hvm64::with_big_stack([&] {
+      hvm64::Book book = hvm64::parse_book(text);
+      hvm64::Host host;
+      host.insert_book(book);
+      h->host = std::move(host);
+    });
+    h->defs_host = h->host.def_records();
+    h->instrs_host = h->host.all_instrs();
+    h->labbits_host = h->host.all_labbits();
+    upload_book(h);
+  });
+}
+
+int hvm_gpu_load_book(hvm_gpu *h, const hvm_book_blob *book) {
+  return guarded(h, [&] {
+    if (!book) throw Error(HVM_E_ARG, "null book");
+    std::vector<std::string> names;
+    std::vector<hvm64::LabSet> labs;
+    std::vector<hvm64::DefCode> code;
+    for (u32 i = 0; i < book->n_defs; i++) {
+      const hvm_def_blob &d = book->defs[i];
+      if (!d.name) throw Error(HVM_E_ARG, "definition without a name");
+      names.push_back(d.name);
+      hvm64::LabSet l;
+      l.min_safe = d.min_safe;
+      l.bits.assign(d.lab_bits, d.lab_bits + d.n_lab_words);
+      labs.push_back(l);
+      hvm64::DefCode c;
+      c.nslots = d.n_slots ? d.n_slots : 1;
+      for (u32 k = 0; k < d.n_instr; k++) {
+        c.instr.push_back(d.instr[k].w0);
+        c.instr.push_back(d.instr[k].w1);
+      }
+      code.push_back(std::move(c));
+    }
+    hvm64::Host host;
+    host.adopt(std::move(names), std::move(labs), std::move(code));
+    h->host = std::move(host);
+    h->defs_host.clear();
+    h->instrs_host.clear();
+    h->labbits_host.clear();
+    for (u32 i = 0; i < book->n_defs; i++) {
+      const hvm_def_blob &d = book->defs[i];
+      DefRec r{};
+      r.ioff = (u32)(h->instrs_host.size() / 2);
+      r.ilen = d.n_instr;
+      r.nslots = d.n_slots ? d.n_slots : 1;
+      r.labs_off = (u32)h->labbits_host.size();
+      r.labs_words = d.n_lab_words;
+      r.min_safe = d.min_safe;
+      for (u32 k = 0; k < d.n_instr; k++) {
+        h->instrs_host.push_back(d.instr[k].w0);
+        h->instrs_host.push_back(d.instr[k].w1);
+      }
+      for (u32 k = 0; k < d.n_lab_words; k++) h->labbits_host.push_back(d.lab_bits[k]);
+      h->defs_host.push_back(r);
+    }
+    upload_book(h);
+  });
+}
+
+int hvm_gpu_def_id(hvm_gpu *h, const char *name, uint32_t *id) {
+  return guarded(h, [&] {
+    if (!name || !id) throw Error(HVM_E_ARG, "null argument");
+    *id = h->host.def_id(name);
+  });
+}
+
+int hvm_gpu_reset(hvm_gpu *h) {
+  return guarded(h, [&] { reset_net(h); });
+}
+
+int hvm_gpu_boot(hvm_gpu *h, uint32_t def_id) {
+  return guarded(h, [&] {
+    if (def_id >= h->ndefs) throw Error(HVM_E_ARG, "no such definition");
+    Params P = make_params(h, ~0ull);
+    hip_check(launch_exec(P, def_id, mkport(T_VAR, 0, h->root_cell), 1, h->stream), "launch boot");
+    sync_globals(h);
+  });
+}
+
+int hvm_gpu_encode_net_text(hvm_gpu *h, const char *net_text) {
+  return guarded(h, [&] {
+    if (!net_text) throw Error(HVM_E_ARG, "null net");
+    hvm64::DefCode code;
+    hvm64::with_big_stack([&] {
+      hvm64::Arena arena;
+      hvm64::Net net = hvm64::parse_net(arena, net_text);
+      code = h->host.encode(arena, net);
+    });
+    // Append the net's code as a temporary definition after the book.
+    DefRec r{};
+    r.ioff = (u32)(h->instrs_host.size() / 2);
+    r.ilen = code.ninstr();
+    r.nslots = code.nslots;
+    r.labs_off = 0;
+    r.labs_words = 0;
+    r.min_safe = 0;
+    std::vector<DefRec> defs = h->defs_host;
+    std::vector<u64> instrs = h->instrs_host;
+    defs.push_back(r);
+    instrs.insert(instrs.end(), code.instr.begin(), code.instr.end());
+    std::swap(defs, h->defs_host);
+    std::swap(instrs, h->instrs_host);
+    upload_book(h);
+    Params P = make_params(h, ~0ull);
+    hip_check(launch_exec(P, (u32)h->defs_host.size() - 1, mkport(T_VAR, 0, h->root_cell), 0, h->stream),
+              "launch encode");
+    sync_globals(h);
+    std::swap(defs, h->defs_host);
+    std::swap(instrs, h->instrs_host);
+    upload_book(h);
+  });
+}
+
+int hvm_gpu_normal(hvm_gpu *h) {
+  return guarded(h, [&] {
+    Params P = make_params(h, ~0ull);
+    hip_check(launch_expand(P, h->root_cell, h->stream), "launch expand");
+    sync_globals(h);
+    h->root_cell = h->last.result;
+    run_reduce(h, ~0ull);
+  });
+}
+
+int hvm_gpu_reduce(hvm_gpu *h, uint64_t limit, uint64_t *popped, int *is_normal) {
+  return guarded(h, [&] {
+    u64 before = h->last.rw[RW_POPS];
+    if (limit == 0) {
+      if (popped) *popped = 0;
+      if (is_normal) *is_normal = (h->last.S >> 32) == 0;
+      return;
+    }
+    run_reduce(h, limit);
+    if (popped) *popped = h->last.rw[RW_POPS] - before;
+    if (is_normal) *is_normal = (h->last.S >> 32) == 0;
+  });
+}
+
+int hvm_gpu_rewrites(hvm_gpu *h, uint64_t out[5]) {
+  return guarded(h, [&] {
+    for (int k = 0; k < 5; k++) out[k] = h->last.rw[k];
+  });
+}
+
+int hvm_gpu_shortcuts(hvm_gpu *h, uint64_t out[3]) {
+  return guarded(h, [&] {
+    for (int k = 0; k < 3; k++) out[k] = h->last.rw[SC_OP_ERA + k];
+  });
+}
+
+int hvm_gpu_readback(hvm_gpu *h, char *buf, uint64_t cap, uint64_t *len) {
+  int rc = 0;
+  int g = guarded(h, [&] {
+    std::string text = readback(h);
+    if (len) *len = text.size();
+    if (!buf || cap < text.size() + 1) {
+      rc = HVM_E_TRUNC;
+      if (buf && cap) {
+        std::memcpy(buf, text.data(), cap - 1);
+        buf[cap - 1] = 0;
+      }
+      return;
+    }
+    std::memcpy(buf, text.data(), text.size());
+    buf[text.size()] = 0;
+  });
+  return g ? g : rc;
+}
+
+int hvm_gpu_last_kernel_ms(hvm_gpu *h, double *ms) {
+  return guarded(h, [&] { *ms = h->last_ms; });
+}
+
+int hvm_gpu_set_timeout(hvm_gpu *h, double seconds) {
+  return guarded(h, [&] {
+    if (seconds < 0) throw Error(HVM_E_ARG, "negative timeout");
+    h->timeout_ticks = (u64)(seconds * (double)h->wall_khz * 1000.0);
+  });
+}
+
+int hvm_gpu_grid(hvm_gpu *h, uint32_t *blocks, uint32_t *waves) {
+  return guarded(h, [&] {
+    if (blocks) *blocks = h->blocks;
+    if (waves) *waves = h->waves;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// Device-side data layout of the reduction engine, shared between the host
+// library and the kernels.
+#pragma once
+#include <stdint.h>
+#include "hvm64_port.h"
+
+namespace hvm {
+
+// Instruction kinds (runtime/src/instruction.rs:324-362).  On the device an
+// instruction is two words:
+//   w0 = kind | a<<16 | b<<32 | c<<48      (a, b, c: trg register slots)
+//   w1 = Const/LinkConst: port; Ctr: label; Op: op; OpNum: op | rhs<<32;
+//        Wires: d (fourth slot)
+enum : u32 { I_CONST = 0, I_LINK, I_LINKCONST, I_CTR, I_OP, I_OPNUM, I_SWITCH, I_WIRES };
+
+HVM_HD u64 instr_w0(u32 kind, u32 a, u32 b, u32 c) {
+  return (u64)kind | ((u64)a << 16) | ((u64)b << 32) | ((u64)c << 48);
+}
+
+// One book definition (def.rs:86-95 + InterpretedDef, def.rs:194-199).
+struct DefRec {
+  u32 ioff;       // first instruction
+  u32 ilen;       // instruction count
+  u32 nslots;     // trg registers needed (after slot compaction)
+  u32 labs_off;   // first word of the label bitset in the label array
+  u32 labs_words; // bitset length in words
+  u32 min_safe;   // LabSet::min_safe (also the label of Ref ports to this def)
+  u32 pad[2];
+};
+
+// Rewrite counter slots (runtime.rs:72-79) and interpreter shortcut counters
+// (ERA targets of do_op / do_op_num / do_switch, instruction.rs:144-197),
+// and the number of redexes popped.
+enum : u32 { RW_ANNI = 0, RW_COMM, RW_ERAS, RW_DREF, RW_OPER, SC_OP_ERA, SC_OPNUM_ERA, SC_SWITCH_ERA, RW_POPS, RW_SLOTS };
+
+// Error codes reported by the device (mirrored in the C ABI header).
+enum : u32 {
+  ERR_NONE = 0, ERR_OOM = 2, ERR_UNIMPL = 3, ERR_DIV0 = 4, ERR_QUEUE = 7, ERR_READBACK = 8,
+  ERR_BADPTR = 12,   // a port addressed memory outside the heap or book (engine bug)
+  ERR_TIMEOUT = 13,  // the kernel ran past its deadline
+};
+
+// Global control block.  Hot words sit on separate 128-byte lines.
+struct alignas(128) Globals {
+  u64 S;           // (queued redexes << 32) | active waves
+  u64 _p0[15];
+  u64 q_head;      // next slot to consume in the global redex ring
+  u64 _p1[15];
+  u64 q_tail;      // next slot to fill
+  u64 _p2[15];
+  u64 ov_top;      // overflow bump pointer (word index)
+  u64 _p3[15];
+  u64 done;        // redexes popped against a reduce(limit) budget
+  u64 _p4[15];
+  u32 err;         // first error code
+  u32 err_info;
+  u64 result;      // kernel-specific result word (new root cell, readback length)
+  u64 _p5[14];
+  u64 rw[RW_SLOTS];  // cumulative counters (RW_POPS: redexes popped)
+  u64 _p6[16 - RW_SLOTS];
+};
+
+// Launch-time parameters of every kernel.
+struct Params {
+  u64 *heap;           // heap[(addr - HVM_HEAP_OFF) >> 3] is the word at engine address `addr`
+  u64 heap_words;
+  u64 lane_node0;      // node index where the per-lane regions start
+  u64 nodes_per_lane;  // nodes per lane (interleaved by 64 within a wave's region)
+  u64 ov_start;        // overflow region [ov_start, ov_end) in words
+  u64 ov_end;
+  const DefRec *defs;
+  const u64 *instrs;   // 2 words per instruction
+  const u64 *labbits;
+  u64 *trg_buf;        // per-wave trg registers: slot-major, 64 lanes interleaved
+  u64 *queue;          // global redex ring, 2 words per slot
+  u64 qmask;           // ring slots - 1
+  Globals *g;
+  u32 ndefs;
+  u32 trg_slots;       // registers per lane
+  u32 total_waves;     // waves in the reduce grid
+  u32 _pad;
+  u64 limit;           // redex budget for reduce(limit); ~0 = unlimited
+  u64 timeout_ticks;   // wall-clock (100 MHz) budget of a launch; 0 = none
+};
+
+// Heap layout: the first RESERVED_WORDS words are fixed cells.
+static const u64 RESERVED_WORDS = 64;
+static const u64 DUMP_WORD = 0;       // 2-word sink written after an error
+static const u64 ROOT0_WORD = 2;      // the root var cell of a fresh net
+
+}  // namespace hvm

@@ -1,0 +1,256 @@
+"""Python mirror of the hvm-64 host/runtime API over the engine's C ABI.
+
+The reference's Rust API (HigherOrderCO/hvm-64 v0.3.0):
+
+    let heap = Heap::new(bytes)                 runtime/src/allocator.rs:22
+    let host = Host::new(&book)                 host/src/host.rs:25
+    let mut net = Net::new(&heap)               runtime/src/net.rs:19
+    net.boot(&host.defs["main"])                runtime/src/net.rs:30
+    net.normal()                                runtime/src/net.rs:55
+    net.reduce(limit) -> Option<usize>          runtime/src/net.rs:41
+    net.rwts                                    runtime/src/runtime.rs:72
+    host.readback(&net)                         host/src/readback.rs:23
+
+maps here to the same names.  Every call runs on the GPU through
+libhvm64_amd.so; there is no CPU fallback: if the library or the device is
+missing, the constructors raise.
+"""
+import ctypes
+import os
+from dataclasses import dataclass
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libhvm64_amd.so")
+
+HVM_E = {
+    1: "PARSE", 2: "OOM", 3: "UNIMPL_PAIR", 4: "DIV0", 5: "UNBOUND", 6: "NOREF",
+    7: "QUEUE", 8: "READBACK", 9: "TRUNC", 10: "ARG", 11: "DEVICE", 12: "TIMEOUT",
+}
+
+EXPORTS = [
+    "hvm_gpu_open", "hvm_gpu_close", "hvm_gpu_last_error", "hvm_gpu_load_book_text", "hvm_gpu_load_book",
+    "hvm_gpu_def_id", "hvm_gpu_reset", "hvm_gpu_boot", "hvm_gpu_encode_net_text", "hvm_gpu_normal",
+    "hvm_gpu_reduce", "hvm_gpu_rewrites", "hvm_gpu_shortcuts", "hvm_gpu_readback", "hvm_gpu_last_kernel_ms",
+    "hvm_gpu_grid", "hvm_gpu_set_timeout",
+]
+
+_lib = None
+
+
+class HvmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{HVM_E.get(code, code)}: {msg}")
+        self.code = code
+        self.kind = HVM_E.get(code, str(code))
+
+
+def load_library(path=None):
+    """Loads libhvm64_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with hvm-64_amd/build.py (or __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    u64 = ctypes.c_uint64
+    lib.hvm_gpu_open.argtypes = [ctypes.c_int, u64, ctypes.POINTER(P)]
+    lib.hvm_gpu_close.argtypes = [P]
+    lib.hvm_gpu_close.restype = None
+    lib.hvm_gpu_last_error.argtypes = [P]
+    lib.hvm_gpu_last_error.restype = ctypes.c_char_p
+    lib.hvm_gpu_load_book_text.argtypes = [P, ctypes.c_char_p]
+    lib.hvm_gpu_def_id.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32)]
+    lib.hvm_gpu_reset.argtypes = [P]
+    lib.hvm_gpu_boot.argtypes = [P, ctypes.c_uint32]
+    lib.hvm_gpu_encode_net_text.argtypes = [P, ctypes.c_char_p]
+    lib.hvm_gpu_normal.argtypes = [P]
+    lib.hvm_gpu_reduce.argtypes = [P, u64, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_int)]
+    lib.hvm_gpu_rewrites.argtypes = [P, ctypes.POINTER(u64)]
+    lib.hvm_gpu_shortcuts.argtypes = [P, ctypes.POINTER(u64)]
+    lib.hvm_gpu_readback.argtypes = [P, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
+    lib.hvm_gpu_last_kernel_ms.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
+    lib.hvm_gpu_grid.argtypes = [P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
+    lib.hvm_gpu_set_timeout.argtypes = [P, ctypes.c_double]
+    _lib = lib
+    return lib
+
+
+def pretty_num(n):
+    """util/src/pretty_num.rs: 1234567 -> '1_234_567'."""
+    s = str(n)
+    parts = []
+    while len(s) > 3:
+        parts.append(s[-3:])
+        s = s[:-3]
+    parts.append(s)
+    return "_".join(reversed(parts))
+
+
+@dataclass
+class Rewrites:
+    """runtime/src/runtime.rs:72-141."""
+    anni: int = 0
+    comm: int = 0
+    eras: int = 0
+    dref: int = 0
+    oper: int = 0
+
+    def total(self):
+        return self.anni + self.comm + self.eras + self.dref + self.oper
+
+    def as_dict(self):
+        return {"rwts": self.total(), "anni": self.anni, "comm": self.comm, "eras": self.eras,
+                "dref": self.dref, "oper": self.oper}
+
+    def __str__(self):
+        rows = [("RWTS   ", self.total()), ("- ANNI ", self.anni), ("- COMM ", self.comm),
+                ("- ERAS ", self.eras), ("- DREF ", self.dref), ("- OPER ", self.oper)]
+        return "".join(f"{k}: {pretty_num(v):>15}\n" for k, v in rows)
+
+
+class Heap:
+    """Device heap + engine handle (Heap::new, runtime/src/allocator.rs:22)."""
+
+    def __init__(self, nbytes=1 << 30, device=0):
+        lib = load_library()
+        h = ctypes.c_void_p()
+        rc = lib.hvm_gpu_open(device, nbytes, ctypes.byref(h))
+        if rc != 0:
+            raise HvmError(rc, lib.hvm_gpu_last_error(None).decode())
+        self._lib = lib
+        self._h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise HvmError(rc, self._lib.hvm_gpu_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self._lib.hvm_gpu_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_timeout(self, seconds):
+        """Wall-clock limit of one reduction launch (HVM_E_TIMEOUT past it)."""
+        self._check(self._lib.hvm_gpu_set_timeout(self._h, float(seconds)))
+
+    def grid(self):
+        b, w = ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(self._lib.hvm_gpu_grid(self._h, ctypes.byref(b), ctypes.byref(w)))
+        return b.value, w.value
+
+
+class DefRef:
+    def __init__(self, name, idx):
+        self.name, self.idx = name, idx
+
+    def __repr__(self):
+        return f"@{self.name}"
+
+
+class _Defs:
+    def __init__(self, host):
+        self._host = host
+
+    def __getitem__(self, name):
+        h = self._host.heap
+        i = ctypes.c_uint32()
+        h._check(h._lib.hvm_gpu_def_id(h._h, name.encode(), ctypes.byref(i)))
+        return DefRef(name, i.value)
+
+    def get(self, name):
+        try:
+            return self[name]
+        except HvmError:
+            return None
+
+
+class Host:
+    """Host::new(&book) (host/src/host.rs:25): loads a book into the device."""
+
+    def __init__(self, book_text, heap):
+        self.heap = heap
+        heap._check(heap._lib.hvm_gpu_load_book_text(heap._h, book_text.encode()))
+        self.defs = _Defs(self)
+
+    def readback(self, net):
+        """Host::readback + Display (host/src/readback.rs:23-33): normal form text."""
+        h = self.heap
+        n = ctypes.c_uint64()
+        cap = 1 << 16
+        while True:
+            buf = ctypes.create_string_buffer(cap)
+            rc = h._lib.hvm_gpu_readback(h._h, buf, cap, ctypes.byref(n))
+            if rc == 9 and n.value + 1 > cap:
+                cap = n.value + 1
+                continue
+            h._check(rc)
+            return buf.value.decode()
+
+    def encode_net(self, net, net_text):
+        """Host::encode_net(net, Var(root), &ast) (host/src/encode.rs:116-120)."""
+        h = self.heap
+        h._check(h._lib.hvm_gpu_encode_net_text(h._h, net_text.encode()))
+
+
+class Net:
+    """Net::new(&heap) (runtime/src/net.rs:19): a fresh net on the heap."""
+
+    def __init__(self, heap):
+        self.heap = heap
+        heap._check(heap._lib.hvm_gpu_reset(heap._h))
+
+    def boot(self, d):
+        self.heap._check(self.heap._lib.hvm_gpu_boot(self.heap._h, d.idx))
+
+    def normal(self):
+        self.heap._check(self.heap._lib.hvm_gpu_normal(self.heap._h))
+
+    def reduce(self, limit):
+        """Returns the number of redexes popped if the net is now normal, else None."""
+        popped = ctypes.c_uint64()
+        normal = ctypes.c_int()
+        h = self.heap
+        h._check(h._lib.hvm_gpu_reduce(h._h, limit, ctypes.byref(popped), ctypes.byref(normal)))
+        return popped.value if normal.value else None
+
+    @property
+    def rwts(self):
+        out = (ctypes.c_uint64 * 5)()
+        self.heap._check(self.heap._lib.hvm_gpu_rewrites(self.heap._h, out))
+        return Rewrites(*[int(x) for x in out])
+
+    @property
+    def shortcuts(self):
+        out = (ctypes.c_uint64 * 3)()
+        self.heap._check(self.heap._lib.hvm_gpu_shortcuts(self.heap._h, out))
+        return tuple(int(x) for x in out)
+
+    def last_kernel_ms(self):
+        ms = ctypes.c_double()
+        self.heap._check(self.heap._lib.hvm_gpu_last_kernel_ms(self.heap._h, ctypes.byref(ms)))
+        return ms.value
+
+
+def normal_with(book_text, entry="main", heap_bytes=1 << 30, heap=None):
+    """tests/loaders.rs:28-38: Host::new + boot(entry) + normal() + readback.
+
+    Returns (Rewrites, normal_form_text, kernel_ms)."""
+    own = heap is None
+    heap = heap or Heap(heap_bytes)
+    try:
+        host = Host(book_text, heap)
+        net = Net(heap)
+        net.boot(host.defs[entry])
+        net.normal()
+        return net.rwts, host.readback(net), net.last_kernel_ms()
+    finally:
+        if own:
+            heap.close()

@@ -1,0 +1,138 @@
+/*
+ * hvm64_amd.h -- C ABI of the MI355X (gfx950) reduction engine for HVM-64
+ * interaction nets.
+ *
+ * The reference (HigherOrderCO/hvm-64 v0.3.0) exposes this path as a Rust
+ * API, not a C ABI; each entry point below names the Rust item it stands in
+ * for.  Conventions: every call returns 0 on success or an HVM_E_* code;
+ * hvm_gpu_last_error() describes the last failure.  Calls are blocking and a
+ * handle must be used by one host thread at a time.  The library owns all
+ * device memory; the caller owns the handle.
+ *
+ * Ports keep the reference's 64-bit layout (runtime/src/port.rs:8-14): tag in
+ * bits 0-2 (Red 0, Var 1, Ref 2, Num 3, Op 5, Switch 6, Ctr 7), address in
+ * bits 3-47, label in bits 48-63, a 32-bit Num in the low half of Num ports.
+ * Addresses are byte offsets in the engine's address space: definition i of
+ * the loaded book is at (i+1)<<5 (so the eraser stays the null Ref, 0x2), and
+ * heap word w is at (1<<32) + 8*w.
+ */
+#ifndef HVM64_AMD_H
+#define HVM64_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define HVM_OK 0
+#define HVM_E_PARSE 1        /* text parse error (ast/src/parser.rs:169-176) */
+#define HVM_E_OOM 2          /* heap exhausted (panic "OOM", runtime/src/allocator.rs:104) */
+#define HVM_E_UNIMPL_PAIR 3  /* Switch-Ctr(0) / Op-Switch (unimplemented!, runtime/src/interact.rs:39-42) */
+#define HVM_E_DIV0 4         /* integer / or % by zero (Rust panic, num/src/num.rs:183-212) */
+#define HVM_E_UNBOUND 5      /* unbound variable (host/src/encode.rs:131) */
+#define HVM_E_NOREF 6        /* reference to a missing definition (host/src/host.rs:35) */
+#define HVM_E_QUEUE 7        /* global redex ring overflow */
+#define HVM_E_READBACK 8     /* malformed net during readback */
+#define HVM_E_TRUNC 9        /* output buffer too small (length reported) */
+#define HVM_E_ARG 10         /* invalid argument */
+#define HVM_E_DEVICE 11      /* HIP runtime error, or an invalid address inside the engine */
+#define HVM_E_TIMEOUT 12     /* a reduction ran past the handle's time limit */
+
+typedef struct hvm_gpu hvm_gpu;
+
+/* One interpreted-definition instruction in device form
+ * (runtime/src/instruction.rs:324-362):
+ *   w0 = kind | a<<16 | b<<32 | c<<48   kind: 0 Const, 1 Link, 2 LinkConst,
+ *        3 Ctr, 4 Op, 5 OpNum, 6 Switch, 7 Wires; a, b, c: trg registers
+ *   w1 = Const/LinkConst: port; Ctr: label; Op: op tag; OpNum: op | rhs<<32;
+ *        Wires: fourth register                                           */
+typedef struct {
+  uint64_t w0, w1;
+} hvm_instr;
+
+/* One definition: Def<InterpretedDef> (runtime/src/def.rs:86-95, 194-199). */
+typedef struct {
+  const char *name;
+  uint32_t min_safe;          /* LabSet::min_safe; label of Ref ports to it */
+  uint32_t n_lab_words;
+  const uint64_t *lab_bits;   /* LabSet::bits */
+  uint32_t n_instr;
+  const hvm_instr *instr;
+  uint32_t n_slots;           /* trg registers (InterpretedDef::trgs, may be compacted) */
+} hvm_def_blob;
+
+typedef struct {
+  uint32_t n_defs;
+  const hvm_def_blob *defs;   /* Ref ports to definition i use address (i+1)<<5 */
+} hvm_book_blob;
+
+/* ~ Heap::new(Some(bytes)) (runtime/src/allocator.rs:22): opens device
+ * `device`, allocates a heap of `heap_bytes` and an empty net. */
+int hvm_gpu_open(int device, uint64_t heap_bytes, hvm_gpu **out);
+
+/* Frees all device memory of the handle. */
+void hvm_gpu_close(hvm_gpu *h);
+
+/* Text of the last error on this handle (or of the last failed open). */
+const char *hvm_gpu_last_error(hvm_gpu *h);
+
+/* ~ Host::new(&book) / Host::insert_book (host/src/host.rs:25-73): parses a
+ * book, computes label sets, encodes every definition and uploads it. */
+int hvm_gpu_load_book_text(hvm_gpu *h, const char *book_text);
+
+/* ~ Host::insert_book for a book already encoded by the caller (for example
+ * the Rust Host's Def<InterpretedDef> list serialised into hvm_def_blob). */
+int hvm_gpu_load_book(hvm_gpu *h, const hvm_book_blob *book);
+
+/* ~ Host::defs[name] (host/src/host.rs:19): index of a loaded definition. */
+int hvm_gpu_def_id(hvm_gpu *h, const char *name, uint32_t *id);
+
+/* ~ Net::new(&heap) (runtime/src/net.rs:19-23): clears the heap, the redex
+ * queue and the rewrite counters; the net's root is a fresh wire. */
+int hvm_gpu_reset(hvm_gpu *h);
+
+/* ~ Net::boot(&def) (runtime/src/net.rs:30-32): calls definition `def_id`
+ * with the root wire (counts one DREF). */
+int hvm_gpu_boot(hvm_gpu *h, uint32_t def_id);
+
+/* ~ Host::encode_net(net, Trg::port(Var(root)), &ast_net)
+ * (host/src/encode.rs:116-120): writes a net given as text into the root. */
+int hvm_gpu_encode_net_text(hvm_gpu *h, const char *net_text);
+
+/* ~ Net::normal() (runtime/src/net.rs:55-60): expand, then reduce to normal
+ * form on the GPU. */
+int hvm_gpu_normal(hvm_gpu *h);
+
+/* ~ Net::reduce(limit) (runtime/src/net.rs:41-52): pops at most `limit`
+ * redexes.  *is_normal = 1 when no redex is left (Some(count)), else 0
+ * (None); *popped = redexes popped. */
+int hvm_gpu_reduce(hvm_gpu *h, uint64_t limit, uint64_t *popped, int *is_normal);
+
+/* ~ Net.rwts (runtime/src/runtime.rs:72-79): anni, comm, eras, dref, oper. */
+int hvm_gpu_rewrites(hvm_gpu *h, uint64_t out[5]);
+
+/* Interpreter shortcuts taken on ERA targets by do_op, do_op_num and
+ * do_switch (runtime/src/instruction.rs:144-146, 163-165, 194-197). */
+int hvm_gpu_shortcuts(hvm_gpu *h, uint64_t out[3]);
+
+/* ~ Host::readback(&net) + Display (host/src/readback.rs:23-33,
+ * ast/src/ast.rs:468-478): normal form as text.  If `cap` is too small,
+ * returns HVM_E_TRUNC with the needed length (without NUL) in *len. */
+int hvm_gpu_readback(hvm_gpu *h, char *buf, uint64_t cap, uint64_t *len);
+
+/* Duration in milliseconds of the last reduction kernel (HIP events on the
+ * kernel's stream), and the reduce grid: blocks and waves. */
+int hvm_gpu_last_kernel_ms(hvm_gpu *h, double *ms);
+
+/* Wall-clock limit of one reduction launch (default 600 s; 0 = none).  A
+ * launch that runs past it stops every wave and returns HVM_E_TIMEOUT. */
+int hvm_gpu_set_timeout(hvm_gpu *h, double seconds);
+int hvm_gpu_grid(hvm_gpu *h, uint32_t *blocks, uint32_t *waves);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HVM64_AMD_H */
