@@ -1,0 +1,170 @@
+"""Benchmark: MIPS of the HIP reduction engine on one MI355X.
+
+A step is one reduction of one net: Net::new (reset) + boot(main) + normal()
+over the workload's book, which is already resident on the device.  Prints
+one JSON line (see the fields below).  Workloads:
+
+  bitonic   bitonic sort of 2**k seeded u24 keys, single net (default, k=16)
+  tree_sum  sum of a complete binary tree of depth d (default d=18)
+  church    Church exponentiation c2 ^ c_e applied to +1 on 0 (default e=22)
+
+The CPU baseline is the oracle (a C++ restatement of the reference evaluator)
+timed on this machine's host cores on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hvm-64_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def workload(name, size, seed):
+    from hvm64 import configs
+    if name == "bitonic":
+        size = 16 if size is None else size
+        return (size, configs.bitonic(size, seed), configs.bitonic_expected(size, seed),
+                f"bitonic sort of 2^{size} u24 keys (seed {seed}), single net")
+    if name == "tree_sum":
+        size = 18 if size is None else size
+        return size, configs.tree_sum(size), configs.tree_sum_expected(size), f"tree-sum depth {size}, single net"
+    if name == "church":
+        size = 22 if size is None else size
+        return (size, configs.church(size, 16), configs.church_expected(size),
+                f"Church c2^c{size} applied to +1 on 0, 16 DUP labels, single net")
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(name, size, seed, seconds_budget=20.0):
+    """Times the oracle's parallel_normal (runtime/src/parallel.rs restated)
+    with T = 2^floor(log2(cores)) threads, cores capped at 16, on a bounded
+    sample: the same workload at a size that takes a few seconds."""
+    try:
+        import oracle_lib
+        from hvm64 import configs
+        o = oracle_lib.Oracle()
+    except Exception as e:  # the oracle is optional for the GPU figure
+        return {"value": None, "unit": "MIPS", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
+    cores = min(16, os.cpu_count() or 1)
+    threads = 1
+    while threads * 2 <= cores:
+        threads *= 2
+    small = {"bitonic": max(size - 3, 6), "tree_sum": max(size - 1, 6), "church": max(size - 2, 4)}[name]
+    text = workload(name, small, seed)[1]
+    single = o.run(text, heap_words=1 << 28, threads=0)
+    multi = o.run(text, heap_words=1 << 28, threads=threads)
+    mips_1 = single["rwts"]["rwts"] / single["seconds"] / 1e6
+    mips_t = multi["rwts"]["rwts"] / multi["seconds"] / 1e6
+    return {"value": round(mips_t, 2), "unit": "MIPS", "cores": threads, "kind": "port",
+            "sample": f"{name} size {small}: {multi['rwts']['rwts']} rewrites, parallel_normal with {threads} threads "
+                      f"({multi['seconds']:.3f} s); single-thread normal() {mips_1:.2f} MIPS ({single['seconds']:.3f} s)",
+            "single_thread_mips": round(mips_1, 2)}
+
+
+def algorithmic_bytes_per_rewrite(name, size, seed):
+    """Heap-word traffic of the reference algorithm per rewrite, counted by
+    the metered oracle (8 B per load or store, 16 B per atomic RMW) on a
+    smaller instance of the workload (the ratio is size-stable)."""
+    try:
+        import oracle_lib
+        o = oracle_lib.Oracle()
+        small = {"bitonic": min(size, 10), "tree_sum": min(size, 12), "church": min(size, 12)}[name]
+        r = o.run(workload(name, small, seed)[1], heap_words=1 << 26, meter=True)
+        m = r["meter"]
+        return (8 * m["loads"] + 8 * m["stores"] + 16 * m["rmws"]) / r["rwts"]["rwts"]
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="bitonic", choices=["bitonic", "tree_sum", "church"])
+    ap.add_argument("--size", type=int, default=None)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--heap-gb", type=float, default=32.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs one net per "
+              f"device and this build has no multi-GPU path).", file=sys.stderr)
+        return 2
+
+    import hvm64
+    size, text, expected, desc = workload(args.workload, args.size, args.seed)
+    heap = hvm64.Heap(int(args.heap_gb * (1 << 30)))
+    heap.set_timeout(300)
+    host = hvm64.Host(text, heap)
+    main_def = host.defs["main"]
+
+    def step():
+        net = hvm64.Net(heap)
+        net.boot(main_def)
+        net.normal()
+        return net
+
+    for _ in range(args.warmup):
+        net = step()
+    t0 = time.perf_counter()
+    kernel_ms = []
+    total = 0
+    for _ in range(args.steps):
+        net = step()
+        kernel_ms.append(net.last_kernel_ms())
+        total += net.rwts.total()
+    elapsed = time.perf_counter() - t0
+    nf = host.readback(net)
+    rw = net.rwts
+    stats = net.sched_stats()
+    ok = nf == expected
+
+    value = total / elapsed / 1e6
+    k_ms = sum(kernel_ms) / len(kernel_ms)
+    bpr = algorithmic_bytes_per_rewrite(args.workload, size, args.seed)
+    roofline = None
+    if bpr is not None:
+        achieved = bpr * rw.total() / (k_ms / 1e3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": None,
+                    "bytes_per_rewrite": round(bpr, 2), "kernel": "k_reduce"}
+    blocks, waves = heap.grid()
+    out = {
+        "metric": "MIPS on 1 MI355X: single net (bitonic sort)",
+        "value": round(value, 2),
+        "unit": "MIPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded in-net key generator)",
+        "config": {"workload": desc, "size": size, "seed": args.seed, "heap_gb": args.heap_gb,
+                   "grid_blocks": blocks, "waves": waves},
+        "rewrites_per_step": rw.total(),
+        "rewrites": rw.as_dict(),
+        "kernel_ms": round(k_ms, 3),
+        "kernel_mips": round(rw.total() / k_ms / 1e3, 2),
+        "result_ok": ok,
+        "sched": stats,
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.workload, size, args.seed)
+    print(json.dumps(out), flush=True)
+    heap.close()
+    return 0 if ok else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -47,6 +48,8 @@ struct hvm_gpu {
   std::vector<DefRec> defs_host;
   std::vector<u64> instrs_host, labbits_host;
   u64 root_cell = 0;
+  u64 ring_node0 = 0, wave_nodes = 0, ov_node0 = 0, ov_nodes = 0;
+  u64 *alloc_pos = nullptr;
   Globals last{};
   double last_ms = 0;
   u64 timeout_ticks = 0;  // wall-clock ticks a reduction may run (0 = unbounded)
@@ -72,13 +75,11 @@ Params make_params(hvm_gpu *h, u64 limit) {
   Params P{};
   P.heap = h->heap;
   P.heap_words = h->heap_words;
-  u64 usable = h->heap_words - RESERVED_WORDS;
-  u64 lane_words = usable / 8 * 7;
-  u64 lanes = (u64)h->waves * 64;
-  P.lane_node0 = RESERVED_WORDS / 2;
-  P.nodes_per_lane = lane_words / 2 / lanes;
-  P.ov_start = RESERVED_WORDS + P.nodes_per_lane * lanes * 2;
-  P.ov_end = h->heap_words;
+  P.ring_node0 = h->ring_node0;
+  P.wave_nodes = h->wave_nodes;
+  P.ov_node0 = h->ov_node0;
+  P.ov_nodes = h->ov_nodes;
+  P.alloc_pos = h->alloc_pos;
   P.defs = h->d_defs;
   P.instrs = h->d_instrs;
   P.labbits = h->d_labbits;
@@ -145,12 +146,16 @@ void upload_book(hvm_gpu *h) {
   ensure_trg(h, slots);
 }
 
-void reset_net(hvm_gpu *h) {
-  hip_check(launch_fill(h->heap, h->heap_words, P_FREE, h->stream), "fill heap");
+// Net::new.  The heap is filled with FREE once, when the handle opens; a
+// reset rewinds every allocation ring instead.  That is enough for a fresh
+// net: a ring's first lap overwrites its slots without reading them, and
+// later laps only reuse slots whose words read FREE, so words left by a
+// previous net are either overwritten or skipped, never read as live.
+void reset_net(hvm_gpu *h, bool fill) {
+  if (fill) hip_check(launch_fill(h->heap, h->heap_words, P_FREE, h->stream), "fill heap");
   hip_check(launch_fill(h->queue, h->qslots * 2, 0, h->stream), "clear ring");
   Globals g0{};
-  Params P = make_params(h, ~0ull);
-  g0.ov_top = P.ov_start;
+  hip_check(hipMemsetAsync(h->alloc_pos, 0, (size_t)h->waves * sizeof(u64), h->stream), "clear rings");
   hip_check(hipMemcpyAsync(h->g, &g0, sizeof(Globals), hipMemcpyHostToDevice, h->stream), "init globals");
   u64 none = P_NONE;
   hip_check(hipMemcpyAsync(h->heap + ROOT0_WORD, &none, sizeof(u64), hipMemcpyHostToDevice, h->stream), "root");
@@ -163,7 +168,9 @@ void reset_net(hvm_gpu *h) {
 void run_reduce(hvm_gpu *h, u64 limit) {
   Params P = make_params(h, limit);
   u64 zero = 0;
+  u64 zst[ST_SLOTS] = {};
   hip_check(hipMemcpyAsync(&h->g->done, &zero, sizeof(u64), hipMemcpyHostToDevice, h->stream), "budget");
+  hip_check(hipMemcpyAsync(h->g->st, zst, sizeof(zst), hipMemcpyHostToDevice, h->stream), "stats");
   hip_check(hipEventRecord(h->ev0, h->stream), "event");
   hip_check(launch_reduce(P, h->blocks, h->stream), "launch reduce");
   hip_check(hipEventRecord(h->ev1, h->stream), "event");
@@ -253,6 +260,19 @@ int hvm_gpu_open(int device, uint64_t heap_bytes, hvm_gpu **out) {
     if (words < min_words) words = min_words;
     h->heap_words = words;
     hip_check(hipMalloc(&h->heap, words * 8), "hipMalloc heap");
+    // Node layout: reserved words, one ring of 2^k nodes per wave (k as large
+    // as fits in 7/8 of the heap), then the shared overflow ring (2^j nodes).
+    u64 nodes = (words - RESERVED_WORDS) / 2;
+    u64 per_wave = 1;
+    while (per_wave * 2 * h->waves <= nodes / 8 * 7) per_wave *= 2;
+    h->ring_node0 = RESERVED_WORDS / 2;
+    h->wave_nodes = per_wave;
+    u64 rest = nodes - per_wave * h->waves;
+    u64 ov = 1;
+    while (ov * 2 <= rest) ov *= 2;
+    h->ov_node0 = h->ring_node0 + per_wave * h->waves;
+    h->ov_nodes = ov;
+    hip_check(hipMalloc(&h->alloc_pos, (size_t)h->waves * sizeof(u64)), "hipMalloc rings");
     hip_check(hipMalloc(&h->g, sizeof(Globals)), "hipMalloc globals");
     h->qslots = 1ull << 22;
     while (h->qslots < (u64)h->waves * 512) h->qslots <<= 1;
@@ -262,7 +282,7 @@ int hvm_gpu_open(int device, uint64_t heap_bytes, hvm_gpu **out) {
     h->wall_khz = wall_khz > 0 ? (u64)wall_khz : 100000;
     h->timeout_ticks = h->wall_khz * 1000 * 600;  // 10 minutes unless the caller sets one
     upload_book(h.get());
-    reset_net(h.get());
+    reset_net(h.get(), true);
   } catch (Error &e) {
     g_open_error = e.what();
     hvm_gpu_close(h.release());
@@ -276,6 +296,7 @@ void hvm_gpu_close(hvm_gpu *h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   dfree(h->heap);
+  dfree(h->alloc_pos);
   dfree(h->g);
   dfree(h->queue);
   dfree(h->d_defs);
@@ -364,7 +385,7 @@ int hvm_gpu_def_id(hvm_gpu *h, const char *name, uint32_t *id) {
 }
 
 int hvm_gpu_reset(hvm_gpu *h) {
-  return guarded(h, [&] { reset_net(h); });
+  return guarded(h, [&] { reset_net(h, false); });
 }
 
 int hvm_gpu_boot(hvm_gpu *h, uint32_t def_id) {
@@ -473,6 +494,73 @@ int hvm_gpu_set_timeout(hvm_gpu *h, double seconds) {
   return guarded(h, [&] {
     if (seconds < 0) throw Error(HVM_E_ARG, "negative timeout");
     h->timeout_ticks = (u64)(seconds * (double)h->wall_khz * 1000.0);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// host-only book utilities (no device needed)
+// ---------------------------------------------------------------------------
+static int host_text(const std::function<std::string()> &fn, char *buf, uint64_t cap, uint64_t *len) {
+  try {
+    std::string text;
+    hvm64::with_big_stack([&] { text = fn(); });
+    if (len) *len = text.size();
+    if (!buf || cap < text.size() + 1) {
+      if (buf && cap) {
+        std::memcpy(buf, text.data(), cap - 1);
+        buf[cap - 1] = 0;
+      }
+      return HVM_E_TRUNC;
+    }
+    std::memcpy(buf, text.data(), text.size() + 1);
+    return 0;
+  } catch (Error &e) {
+    g_open_error = e.what();
+    return e.code;
+  } catch (std::exception &e) {
+    g_open_error = e.what();
+    return HVM_E_ARG;
+  }
+}
+
+int hvm_book_print(const char *book_text, char *buf, uint64_t cap, uint64_t *len) {
+  if (!book_text) return HVM_E_ARG;
+  return host_text([&] { return hvm64::show_book(hvm64::parse_book(book_text)); }, buf, cap, len);
+}
+
+int hvm_book_label_sets(const char *book_text, char *buf, uint64_t cap, uint64_t *len) {
+  if (!book_text) return HVM_E_ARG;
+  return host_text(
+      [&] {
+        auto sets = hvm64::calculate_label_sets(hvm64::parse_book(book_text));
+        std::string out;
+        for (auto &kv : sets) {
+          out += kv.first + ":";
+          for (u32 l = 0; l < kv.second.min_safe; l++)
+            if (kv.second.has((u16)l)) out += " " + std::to_string(l);
+          out += "\n";
+        }
+        return out;
+      },
+      buf, cap, len);
+}
+
+int hvm_net_apply_tree(const char *net_text, const char *tree_text, char *buf, uint64_t cap, uint64_t *len) {
+  if (!net_text || !tree_text) return HVM_E_ARG;
+  return host_text(
+      [&] {
+        hvm64::Arena arena;
+        hvm64::Net net = hvm64::parse_net(arena, net_text);
+        u32 arg = hvm64::parse_tree(arena, tree_text);
+        hvm64::apply_tree(arena, net, arg);
+        return hvm64::show_net(arena, net);
+      },
+      buf, cap, len);
+}
+
+int hvm_gpu_sched_stats(hvm_gpu *h, uint64_t out[7]) {
+  return guarded(h, [&] {
+    for (u32 k = 0; k < ST_SLOTS; k++) out[k] = h->last.st[k];
   });
 }
 

@@ -33,6 +33,11 @@ struct DefRec {
 // and the number of redexes popped.
 enum : u32 { RW_ANNI = 0, RW_COMM, RW_ERAS, RW_DREF, RW_OPER, SC_OP_ERA, SC_OPNUM_ERA, SC_SWITCH_ERA, RW_POPS, RW_SLOTS };
 
+// Scheduler statistics (summed over waves): iterations with work, idle
+// iterations, wall-clock ticks (100 MHz) spent in iterations with work,
+// spills to the global ring / redexes spilled, claims / redexes claimed.
+enum : u32 { ST_BUSY_IT = 0, ST_IDLE_IT, ST_BUSY_TICKS, ST_SPILLS, ST_SPILLED, ST_CLAIMS, ST_CLAIMED, ST_SLOTS };
+
 // Error codes reported by the device (mirrored in the C ABI header).
 enum : u32 {
   ERR_NONE = 0, ERR_OOM = 2, ERR_UNIMPL = 3, ERR_DIV0 = 4, ERR_QUEUE = 7, ERR_READBACK = 8,
@@ -48,7 +53,7 @@ struct alignas(128) Globals {
   u64 _p1[15];
   u64 q_tail;      // next slot to fill
   u64 _p2[15];
-  u64 ov_top;      // overflow bump pointer (word index)
+  u64 ov_pos;      // allocations taken from the shared overflow ring
   u64 _p3[15];
   u64 done;        // redexes popped against a reduce(limit) budget
   u64 _p4[15];
@@ -58,16 +63,18 @@ struct alignas(128) Globals {
   u64 _p5[14];
   u64 rw[RW_SLOTS];  // cumulative counters (RW_POPS: redexes popped)
   u64 _p6[16 - RW_SLOTS];
+  u64 st[ST_SLOTS];  // scheduler statistics of the last reduce launch
 };
 
 // Launch-time parameters of every kernel.
 struct Params {
   u64 *heap;           // heap[(addr - HVM_HEAP_OFF) >> 3] is the word at engine address `addr`
   u64 heap_words;
-  u64 lane_node0;      // node index where the per-lane regions start
-  u64 nodes_per_lane;  // nodes per lane (interleaved by 64 within a wave's region)
-  u64 ov_start;        // overflow region [ov_start, ov_end) in words
-  u64 ov_end;
+  u64 ring_node0;      // node index of wave 0's allocation ring
+  u64 wave_nodes;      // nodes per wave ring (a power of two)
+  u64 ov_node0;        // node index of the shared overflow ring
+  u64 ov_nodes;        // nodes in the overflow ring (a power of two)
+  u64 *alloc_pos;      // per-wave ring positions, persisted across launches
   const DefRec *defs;
   const u64 *instrs;   // 2 words per instruction
   const u64 *labbits;

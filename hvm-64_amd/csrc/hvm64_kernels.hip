@@ -50,7 +50,7 @@ __global__ void k_fill(u64 *words, u64 n, u64 value) {
 __global__ void k_exec(Params P, u32 def, u64 t0, u32 count_dref) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   Lane L;
-  lane_init(L, P, 0, 0);
+  lane_init(L, P, 0, 0, &P.alloc_pos[0]);
   if (count_dref) L.rw[RW_DREF]++;
   interpret(L, def, t0);
   publish();
@@ -65,7 +65,7 @@ __global__ void k_exec(Params P, u32 def, u64 t0, u32 count_dref) {
 __global__ void k_expand(Params P, u64 old_root_cell) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   Lane L;
-  lane_init(L, P, 0, 0);
+  lane_init(L, P, 0, 0, &P.alloc_pos[0]);
   u64 r2 = alloc_var(L);
   u64 e = alloc_var(L);
   st(P, e, var_port(r2));
@@ -79,83 +79,116 @@ __global__ void k_expand(Params P, u64 old_root_cell) {
 
 // The persistent reduction kernel (net.rs:41-60).  Every wave owns an LDS
 // ring of redexes; each iteration its 64 lanes pop up to 64 redexes (LIFO)
-// and interact them; new redexes are pushed back into the ring.  Rings spill
-// their oldest half to the global ring while other waves are idle, and an
-// empty wave claims work from the global ring.  S = (queued << 32) | active;
-// the reduction is complete when S == 0, a state no wave can leave.
+// and interact them; new redexes are pushed back into the ring.  A wave that
+// ends an iteration with more than one iteration's worth of redexes spills
+// the excess (its oldest redexes) to the global ring while other waves are
+// idle; an empty wave claims up to 64 redexes from the global ring.
+// S = (queued << 32) | active waves; the reduction is complete when S == 0,
+// a state no wave can leave.  S is one cache line shared by every wave, so
+// busy waves touch it only to spill (and read it every 16 iterations), and
+// idle waves poll it with a growing, jittered backoff.
+static const u32 REFRESH = 16;      // busy iterations between control-word reads
+static const u32 MAX_BACKOFF = 1024;  // idle backoff cap, in s_sleep(2) units (~50 us)
+
+DEV void wave_sleep(u32 units) {
+  for (u32 z = 0; z < units; z++) __builtin_amdgcn_s_sleep(2);
+}
+
 __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   __shared__ u64 s_bag[WAVES_PER_BLOCK][BAG_CAP * 2];
+  __shared__ u64 s_T[WAVES_PER_BLOCK][LDS_SLOTS * 64];
+  __shared__ u64 s_alloc[WAVES_PER_BLOCK];
   __shared__ u32 s_top[WAVES_PER_BLOCK];
   __shared__ u32 s_bot[WAVES_PER_BLOCK];
   const u32 wib = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const u32 gwave = blockIdx.x * WAVES_PER_BLOCK + wib;
   Globals *g = P.g;
   u64 *bag = s_bag[wib];
-  Lane L;
-  lane_init(L, P, gwave, lane);
-  L.bag = bag;
-  L.bag_top = &s_top[wib];
   if (lane == 0) {
     s_top[wib] = 0;
     s_bot[wib] = 0;
+    s_alloc[wib] = P.alloc_pos[gwave];
   }
+  __builtin_amdgcn_wave_barrier();
+  Lane L;
+  lane_init(L, P, gwave, lane, &s_alloc[wib]);
+  L.T_lds = &s_T[wib][lane];
+  L.bag = bag;
+  L.bag_top = &s_top[wib];
   __builtin_amdgcn_wave_barrier();
   bool active = false;
   const bool limited = P.limit != ~0ull;
-  u32 idle_sleep = 1;
   const u64 t_end = P.timeout_ticks ? wall_clock64() + P.timeout_ticks : ~0ull;
+  u32 backoff = 1 + (gwave & 7);
+  u32 busy = 0;                // busy iterations since the last control read
+  u64 S_cache = 0;             // last value of S seen by this wave
+  u64 S_next = 0;              // lane 0: control words read at iteration end
+  u32 err_next = 0;
+  bool fresh = false;          // S_next / err_next hold a read to consume
+  u64 st[ST_SLOTS] = {0, 0, 0, 0, 0, 0, 0};
 
   for (;;) {
-    // ---- control words (one lane reads, the wave shares)
-    u64 S = 0;
-    u32 err = 0;
-    if (lane == 0) {
-      err = gld32(&g->err);
-      S = gld(&g->S);
+    if (fresh) {
+      S_cache = __shfl(S_next, 0, 64);
+      u32 err = __shfl(err_next, 0, 64);
+      fresh = false;
+      if (err) break;
+      if (wall_clock64() > t_end) {
+        fail(L, ERR_TIMEOUT);
+        break;
+      }
     }
-    err = __shfl(err, 0, 64);
-    S = __shfl(S, 0, 64);
-    if (err) break;
-    if (wall_clock64() > t_end) {
-      fail(L, ERR_TIMEOUT);
-      break;
-    }
-
     u32 top = s_top[wib], bot = s_bot[wib];
     u32 n = top - bot;
 
-    // ---- claim work from the global ring when the local ring is empty
+    // ---- empty ring: claim work from the global ring, or idle
     if (n == 0) {
       u32 k = 0;
+      u64 s = 0;
+      u32 err = 0;
       if (lane == 0) {
-        u64 s = S;
-        for (int tries = 0; tries < 4; tries++) {
+        s = gld(&g->S);
+        err = gld32(&g->err);
+        for (int tries = 0; tries < 4 && !err; tries++) {
           u64 q = s >> 32;
           if (q == 0) break;
-          u64 want = q > 128 ? 64 : (q + 1) / 2;
+          u64 want = q < 64 ? q : 64;
           u64 ns = s - (want << 32) + (active ? 0 : 1);
           if (__hip_atomic_compare_exchange_strong(&g->S, &s, ns, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_AGENT)) {
             k = (u32)want;
+            s = ns;
             break;
           }
         }
       }
       k = __shfl(k, 0, 64);
+      s = __shfl(s, 0, 64);
+      err = __shfl(err, 0, 64);
+      if (err) break;
+      S_cache = s;
       if (k == 0) {
+        st[ST_IDLE_IT]++;
         if (active) {
-          if (lane == 0)
-            __hip_atomic_fetch_sub(&g->S, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          u64 old = 0;
+          if (lane == 0) old = __hip_atomic_fetch_sub(&g->S, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          s = __shfl(old, 0, 64) - 1;
           active = false;
         }
-        if (S == 0) break;
+        if (s == 0) break;
         if (limited && gld(&g->done) >= P.limit) break;
-        for (u32 z = 0; z < idle_sleep; z++) __builtin_amdgcn_s_sleep(2);
-        idle_sleep = idle_sleep < 64 ? idle_sleep * 2 : 64;
+        if (wall_clock64() > t_end) {
+          fail(L, ERR_TIMEOUT);
+          break;
+        }
+        wave_sleep(backoff);
+        backoff = backoff < MAX_BACKOFF ? backoff * 2 + (gwave & 3) : MAX_BACKOFF - (gwave & 63);
         continue;
       }
-      idle_sleep = 1;
+      backoff = 1 + (gwave & 7);
       active = true;
+      st[ST_CLAIMS]++;
+      st[ST_CLAIMED] += k;
       u64 base = 0;
       if (lane == 0) base = __hip_atomic_fetch_add(&g->q_head, (u64)k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       base = __shfl(base, 0, 64);
@@ -192,7 +225,10 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
       if (lane == 0) {
         u64 d = gld(&g->done);
         for (;;) {
-          if (d >= P.limit) { granted = 0; break; }
+          if (d >= P.limit) {
+            granted = 0;
+            break;
+          }
           u64 want = P.limit - d < take ? P.limit - d : take;
           if (__hip_atomic_compare_exchange_strong(&g->done, &d, d + want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_AGENT)) {
@@ -204,20 +240,19 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
       take = __shfl(granted, 0, 64);
       if (take == 0) {
         // budget spent: hand the whole ring back to the global queue and stop
+        u64 base = 0;
+        if (lane == 0) base = __hip_atomic_fetch_add(&g->q_tail, (u64)n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        base = __shfl(base, 0, 64);
         for (u32 i = lane; i < n; i += 64) {
-          u32 s = ((bot + i) & (BAG_CAP - 1)) * 2;
-          u64 a = bag[s], b = bag[s + 1];
-          publish();
-          u64 pos = __hip_atomic_fetch_add(&g->q_tail, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          u64 *q = P.queue + (pos & P.qmask) * 2;
-          gst(q, a);
-          gst(q + 1, b);
+          u32 s2 = ((bot + i) & (BAG_CAP - 1)) * 2;
+          u64 *q = P.queue + ((base + i) & P.qmask) * 2;
+          gst(q, bag[s2]);
+          gst(q + 1, bag[s2 + 1]);
         }
         publish();
         if (lane == 0) {
           s_bot[wib] = top;
-          __hip_atomic_fetch_add(&g->S, ((u64)n << 32) - (active ? 1 : 0), __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(&g->S, ((u64)n << 32) - (active ? 1 : 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         active = false;
         break;
@@ -225,6 +260,8 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     }
 
     // ---- pop (LIFO) and interact
+    const u64 t_it = wall_clock64();
+    st[ST_BUSY_IT]++;
     bool has = lane < take;
     u64 ra = 0, rb = 0;
     if (has) {
@@ -240,33 +277,47 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
       L.rw[RW_POPS]++;
       interact(L, ra, rb);
     }
+    if (++busy >= REFRESH) {  // in flight while this iteration's stores retire
+      busy = 0;
+      fresh = true;
+      if (lane == 0) {
+        S_next = gld(&g->S);
+        err_next = gld32(&g->err);
+      }
+    }
     publish();
+    L.dirty = false;
     __builtin_amdgcn_wave_barrier();
 
-    // ---- clamp the ring after overflow pushes, then balance
+    // ---- clamp the ring after overflow pushes, then spill the excess
     top = s_top[wib];
     if (top - bot > BAG_CAP) top = bot + BAG_CAP;
     n = top - bot;
-    u32 idle_waves = P.total_waves - (u32)(S & 0xFFFFFFFFu);
-    u32 queued = (u32)(S >> 32);
+    u32 active_waves = (u32)(S_cache & 0xFFFFFFFFu);
+    u32 idle_waves = P.total_waves > active_waves ? P.total_waves - active_waves : 0;
+    u32 queued = (u32)(S_cache >> 32);
     u32 spill = 0;
     if (n > BAG_CAP * 3 / 4) spill = n - BAG_CAP / 2;  // keep room for one iteration's pushes
-    else if (n >= 2 && idle_waves > 0 && queued < idle_waves * 2) spill = n / 2;
+    else if (n > 64 + 16 && idle_waves > 0 && queued < idle_waves * 64) spill = n - 64;
     if (spill) {
+      st[ST_SPILLS]++;
+      st[ST_SPILLED] += spill;
       u64 base = 0;
       if (lane == 0) base = __hip_atomic_fetch_add(&g->q_tail, (u64)spill, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       base = __shfl(base, 0, 64);
       for (u32 i = lane; i < spill; i += 64) {
-        u32 s = ((bot + i) & (BAG_CAP - 1)) * 2;
+        u32 s2 = ((bot + i) & (BAG_CAP - 1)) * 2;
         u64 *q = P.queue + ((base + i) & P.qmask) * 2;
-        gst(q, bag[s]);
-        gst(q + 1, bag[s + 1]);
+        gst(q, bag[s2]);
+        gst(q + 1, bag[s2 + 1]);
       }
       publish();
+      u64 old = 0;
       if (lane == 0) {
-        u64 old = __hip_atomic_fetch_add(&g->S, (u64)spill << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        old = __hip_atomic_fetch_add(&g->S, (u64)spill << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if ((old >> 32) + spill > (P.qmask + 1) / 2) fail(L, ERR_QUEUE);
       }
+      S_cache = __shfl(old, 0, 64) + ((u64)spill << 32);
       bot += spill;
     }
     if (lane == 0) {
@@ -274,11 +325,18 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
       s_bot[wib] = bot;
     }
     __builtin_amdgcn_wave_barrier();
+    st[ST_BUSY_TICKS] += wall_clock64() - t_it;
   }
 
   lane_finish(L);
   publish();
   wave_flush_counters(L, lane);
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) {
+    P.alloc_pos[gwave] = s_alloc[wib];
+    for (u32 k = 0; k < ST_SLOTS; k++)
+      if (st[k]) __hip_atomic_fetch_add(&g->st[k], st[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // Serialise the tree hanging from `root_cell` in preorder (readback.rs:288-365

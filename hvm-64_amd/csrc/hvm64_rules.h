@@ -6,6 +6,7 @@
 
 #if defined(__HIPCC__) || defined(__HIP__)
 #define DEV __device__ __forceinline__
+#define NOINL __device__ __noinline__
 #define HVM_LD(p) __hip_atomic_load((p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
 #define HVM_ST(p, v) __hip_atomic_store((p), (v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
 #define HVM_XCHG(p, v) __hip_atomic_exchange((p), (v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
@@ -17,6 +18,7 @@
 #define HVM_PUBLISH() __builtin_amdgcn_s_waitcnt(0x0F70)
 #else
 #define DEV static inline
+#define NOINL static
 #define HVM_LD(p) __atomic_load_n((p), __ATOMIC_RELAXED)
 #define HVM_ST(p, v) __atomic_store_n((p), (v), __ATOMIC_RELAXED)
 #define HVM_XCHG(p, v) __atomic_exchange_n((p), (v), __ATOMIC_RELAXED)
@@ -31,7 +33,8 @@ namespace hvm {
 
 
 static const u32 WAVES_PER_BLOCK = 4;
-static const u32 BAG_CAP = 256;  // redexes per wave in LDS (power of two)
+static const u32 BAG_CAP = 256;    // redexes per wave in LDS (power of two)
+static const u32 LDS_SLOTS = 10;   // trg registers per lane kept in LDS
 
 // ---------------------------------------------------------------------------
 // heap access
@@ -66,15 +69,14 @@ DEV u64 norm_endpoint(u64 p) { return is_endpoint(p) ? (paddr(p) | T_VAR) : p; }
 // ---------------------------------------------------------------------------
 struct Lane {
   const Params *P;
-  // allocator: this lane owns nodes lane_node0 + (region*nodes_per_lane + k)*64 + lane
-  u64 my_node0;
-  u64 cursor;
-  u64 nslots;
-  u64 ov_cur, ov_end;  // overflow bump chunk (word indices)
+  // allocator: the wave's ring of nodes [wave_node0, wave_node0 + P->wave_nodes)
+  u64 wave_node0;
+  u64 *alloc_ctr;      // the ring's position counter (LDS in the reduce kernel)
   u64 spare;           // reserved second word of a var pair, or 0
-  bool wrapped, ov_mode;
-  // trg registers
-  u64 *T;
+  bool dirty;          // stores of new nodes/cells not yet retired
+  // trg registers: T[s * 64] (LDS for small definitions, else global)
+  u64 *T_lds;
+  u64 *T_glb;
   // LDS redex bag (nullptr: push straight to the global ring)
   u64 *bag;
   u32 *bag_top;
@@ -90,37 +92,40 @@ DEV void fail(Lane &L, u32 code) {
 
 DEV u64 dump_addr() { return HVM_HEAP_OFF + DUMP_WORD * 8; }
 
-// Node allocation: scan this lane's interleaved slots for a free node; after
-// one full wrap, bump-allocate 64-node chunks from the shared overflow region.
-DEV u64 alloc_node(Lane &L) {
+// Ring allocation.  Every wave owns a ring of 2^k nodes; the ring position
+// counts allocations since the heap was reset, so on the first lap every slot
+// is known to be free and is taken without reading it.  Later laps take the
+// next slot whose two words are FREE (frees write FREE in place, from any
+// wave).  A ring with no free slot in a full lap falls back to the shared
+// overflow ring, which works the same way.
+DEV u64 ring_take(Lane &L, u64 *ctr, u64 node0, u64 nodes, bool &ok) {
   const Params &P = *L.P;
-  while (!L.ov_mode) {
-    if (L.cursor >= L.nslots) {
-      if (L.wrapped || L.nslots == 0) {
-        L.ov_mode = true;
-        break;
-      }
-      L.wrapped = true;
-      L.cursor = 0;
+  for (u64 miss = 0; miss <= nodes; miss++) {
+    u64 pos = HVM_FAA(ctr, 1ull);
+    u64 addr = HVM_HEAP_OFF + (node0 + (pos & (nodes - 1))) * 16;
+    if (pos < nodes) {
+      ok = true;
+      return addr;
     }
-    u64 addr = HVM_HEAP_OFF + (L.my_node0 + L.cursor * 64) * 16;
-    L.cursor++;
     u64 w0 = ld(P, addr), w1 = ld(P, addr + 8);
-    if (w0 == P_FREE && w1 == P_FREE) return addr;
-  }
-  if (L.ov_cur >= L.ov_end) {
-    const u64 chunk = 128;  // words
-    u64 start = HVM_FAA(&P.g->ov_top, chunk);
-    if (start + chunk > P.ov_end) {
-      fail(L, ERR_OOM);
-      return dump_addr();
+    if (w0 == P_FREE && w1 == P_FREE) {
+      ok = true;
+      return addr;
     }
-    L.ov_cur = start;
-    L.ov_end = start + chunk;
   }
-  u64 addr = HVM_HEAP_OFF + L.ov_cur * 8;
-  L.ov_cur += 2;
-  return addr;
+  ok = false;
+  return dump_addr();
+}
+
+NOINL u64 alloc_node(Lane &L) {
+  const Params &P = *L.P;
+  bool ok;
+  u64 a = ring_take(L, L.alloc_ctr, L.wave_node0, P.wave_nodes, ok);
+  if (ok) return a;
+  a = ring_take(L, &P.g->ov_pos, P.ov_node0, P.ov_nodes, ok);
+  if (ok) return a;
+  fail(L, ERR_OOM);
+  return dump_addr();
 }
 
 // Var cells come in pairs from one node; both words are reserved at once.
@@ -133,6 +138,7 @@ DEV u64 alloc_var(Lane &L) {
   u64 n = alloc_node(L);
   st(*L.P, n, P_NONE);
   st(*L.P, n + 8, P_NONE);
+  L.dirty = true;
   L.spare = n + 8;
   return n;
 }
@@ -141,6 +147,7 @@ DEV u64 new_node(Lane &L, u32 tag, u32 lab, u64 p1, u64 p2) {
   u64 n = alloc_node(L);
   st(*L.P, n, norm_endpoint(p1));
   st(*L.P, n + 8, norm_endpoint(p2));
+  L.dirty = true;
   return mkport(tag, lab, n);
 }
 
@@ -153,12 +160,20 @@ DEV void take(Lane &L, u64 p, u64 &x, u64 &y) {
   st(*L.P, a + 8, P_FREE);
 }
 
+// Retire this lane's stores before a port to new memory becomes visible.
+DEV void publish_if_dirty(Lane &L) {
+  if (L.dirty) {
+    publish();
+    L.dirty = false;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // redexes
 // ---------------------------------------------------------------------------
-DEV void global_push1(Lane &L, u64 a, u64 b) {
+NOINL void global_push1(Lane &L, u64 a, u64 b) {
   const Params &P = *L.P;
-  publish();
+  publish_if_dirty(L);
   u64 pos = HVM_FAA(&P.g->q_tail, 1ull);
   u64 *s = P.queue + (pos & P.qmask) * 2;
   gst(s, a);
@@ -186,27 +201,38 @@ DEV void redux(Lane &L, u64 a, u64 b) {
   global_push1(L, a, b);
 }
 
-// Connect two endpoints/ports.  Each step consumes one var cell, so a chain
-// longer than the heap can only come from a corrupted net: stop with an error.
+// Linking.  A link step either meets two principal ports (a redex) or swaps a
+// port into the var cell of an endpoint: the first endpoint to arrive leaves
+// its port there; the second finds it, frees the cell and links on.  Each
+// step consumes a var cell, so a longer chain means a corrupted net.
 static const u32 MAX_CHAIN = 1u << 24;
-DEV void link(Lane &L, u64 a, u64 b) {
+
+// First step: returns true with `cell` set when a swap into `cell` of
+// norm_endpoint(b) is pending, false when the link is already complete.
+DEV bool link_begin(Lane &L, u64 &a, u64 &b, u64 &cell) {
+  if (!is_endpoint(a)) {
+    if (!is_endpoint(b)) {
+      redux(L, a, b);
+      return false;
+    }
+    u64 t = a;
+    a = b;
+    b = t;
+  }
+  cell = paddr(a);
+  return true;
+}
+
+NOINL void link(Lane &L, u64 a, u64 b) {
   const Params &P = *L.P;
   for (u32 hop = 0;; hop++) {
     if (hop > MAX_CHAIN) {
       fail(L, ERR_BADPTR);
       return;
     }
-    if (!is_endpoint(a)) {
-      if (!is_endpoint(b)) {
-        redux(L, a, b);
-        return;
-      }
-      u64 t = a;
-      a = b;
-      b = t;
-    }
-    u64 cell = paddr(a);
-    publish();
+    u64 cell;
+    if (!link_begin(L, a, b, cell)) return;
+    publish_if_dirty(L);
     u64 old = xchg(P, cell, norm_endpoint(b));
     if (old == P_NONE) return;
     st(P, cell, P_FREE);
@@ -214,10 +240,47 @@ DEV void link(Lane &L, u64 a, u64 b) {
   }
 }
 
+// Two / four independent links whose first swaps are in flight together.
+DEV void link2(Lane &L, u64 a1, u64 b1, u64 a2, u64 b2) {
+  const Params &P = *L.P;
+  u64 c1 = 0, c2 = 0;
+  bool p1 = link_begin(L, a1, b1, c1);
+  bool p2 = link_begin(L, a2, b2, c2);
+  if (p1 || p2) publish_if_dirty(L);
+  u64 o1 = p1 ? xchg(P, c1, norm_endpoint(b1)) : P_NONE;
+  u64 o2 = p2 ? xchg(P, c2, norm_endpoint(b2)) : P_NONE;
+  if (o1 != P_NONE) {
+    st(P, c1, P_FREE);
+    link(L, o1, b1);
+  }
+  if (o2 != P_NONE) {
+    st(P, c2, P_FREE);
+    link(L, o2, b2);
+  }
+}
+
+DEV void link4(Lane &L, u64 a1, u64 b1, u64 a2, u64 b2, u64 a3, u64 b3, u64 a4, u64 b4) {
+  const Params &P = *L.P;
+  u64 c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+  bool p1 = link_begin(L, a1, b1, c1);
+  bool p2 = link_begin(L, a2, b2, c2);
+  bool p3 = link_begin(L, a3, b3, c3);
+  bool p4 = link_begin(L, a4, b4, c4);
+  if (p1 || p2 || p3 || p4) publish_if_dirty(L);
+  u64 o1 = p1 ? xchg(P, c1, norm_endpoint(b1)) : P_NONE;
+  u64 o2 = p2 ? xchg(P, c2, norm_endpoint(b2)) : P_NONE;
+  u64 o3 = p3 ? xchg(P, c3, norm_endpoint(b3)) : P_NONE;
+  u64 o4 = p4 ? xchg(P, c4, norm_endpoint(b4)) : P_NONE;
+  if (o1 != P_NONE) { st(P, c1, P_FREE); link(L, o1, b1); }
+  if (o2 != P_NONE) { st(P, c2, P_FREE); link(L, o2, b2); }
+  if (o3 != P_NONE) { st(P, c3, P_FREE); link(L, o3, b3); }
+  if (o4 != P_NONE) { st(P, c4, P_FREE); link(L, o4, b4); }
+}
+
 // Follow a wire endpoint whose other side has already linked (the
 // reference's Trg::target on a wire, linker.rs:401-404).  The caller owns the
 // endpoint, so once a cell holds a port the cell is ours to free.
-DEV u64 resolve(Lane &L, u64 p) {
+NOINL u64 resolve(Lane &L, u64 p) {
   const Params &P = *L.P;
   for (u32 hop = 0; is_endpoint(p); hop++) {
     if (hop > MAX_CHAIN) {
@@ -238,16 +301,15 @@ DEV u64 as_wire(u64 p) { return is_endpoint(p) ? wire_port(paddr(p)) : p; }
 // ---------------------------------------------------------------------------
 // interaction rules (interact.rs)
 // ---------------------------------------------------------------------------
-DEV void anni2(Lane &L, u64 a, u64 b) {
+NOINL void anni2(Lane &L, u64 a, u64 b) {
   L.rw[RW_ANNI]++;
   u64 a1, a2, b1, b2;
   take(L, a, a1, a2);
   take(L, b, b1, b2);
-  link(L, a1, b1);
-  link(L, a2, b2);
+  link2(L, a1, b1, a2, b2);
 }
 
-DEV void comm22(Lane &L, u64 a, u64 b) {
+NOINL void comm22(Lane &L, u64 a, u64 b) {
   L.rw[RW_COMM]++;
   u64 a1, a2, b1, b2;
   take(L, a, a1, a2);
@@ -257,21 +319,17 @@ DEV void comm22(Lane &L, u64 a, u64 b) {
   u64 A2 = new_node(L, ptag(a), plab(a), var_port(v3), var_port(v4));
   u64 B1 = new_node(L, ptag(b), plab(b), var_port(v1), var_port(v3));
   u64 B2 = new_node(L, ptag(b), plab(b), var_port(v2), var_port(v4));
-  link(L, a1, B1);
-  link(L, a2, B2);
-  link(L, b1, A1);
-  link(L, b2, A2);
+  link4(L, a1, B1, a2, B2, b1, A1, b2, A2);
 }
 
-DEV void comm02(Lane &L, u64 a, u64 b) {
+NOINL void comm02(Lane &L, u64 a, u64 b) {
   L.rw[RW_COMM]++;
   u64 b1, b2;
   take(L, b, b1, b2);
-  link(L, b1, a);
-  link(L, b2, a);
+  link2(L, b1, a, b2, a);
 }
 
-DEV void switch_num(Lane &L, u64 a, u64 b) {  // interact.rs:201-223
+NOINL void switch_num(Lane &L, u64 a, u64 b) {  // interact.rs:201-223
   L.rw[RW_OPER]++;
   u64 a1, a2;
   take(L, a, a1, a2);
@@ -287,7 +345,7 @@ DEV void switch_num(Lane &L, u64 a, u64 b) {  // interact.rs:201-223
   }
 }
 
-DEV void op_num(Lane &L, u64 a, u64 b) {  // interact.rs:251-270
+NOINL void op_num(Lane &L, u64 a, u64 b) {  // interact.rs:251-270
   u32 op = plab(a);
   u64 a1, a2;
   take(L, a, a1, a2);
@@ -314,11 +372,9 @@ DEV bool labs_has(const Params &P, const DefRec &d, u32 lab) {  // def.rs:30-37
   return (P.labbits[d.labs_off + idx] >> (lab & 63)) & 1;
 }
 
-DEV u64 T_get(Lane &L, u32 s) { return L.T[(u64)s * 64]; }
-DEV void T_set(Lane &L, u32 s, u64 v) { L.T[(u64)s * 64] = v; }
 
 // instruction.rs:403-422
-DEV void do_ctr(Lane &L, u32 lab, u64 trg, u64 &o1, u64 &o2) {
+NOINL void do_ctr(Lane &L, u32 lab, u64 trg, u64 &o1, u64 &o2) {
   u64 port = ptag(trg) == T_RED ? resolve(L, trg) : trg;
   if (ptag(port) == T_CTR && plab(port) == lab) {
     u64 x, y;
@@ -336,7 +392,7 @@ DEV void do_ctr(Lane &L, u32 lab, u64 trg, u64 &o1, u64 &o2) {
 }
 
 // instruction.rs:425-442
-DEV void do_op(Lane &L, u32 op, u64 trg, u64 &o1, u64 &o2) {
+NOINL void do_op(Lane &L, u32 op, u64 trg, u64 &o1, u64 &o2) {
   u64 port = ptag(trg) == T_RED ? resolve(L, trg) : trg;
   if (ptag(port) == T_NUM) {
     u64 v = alloc_var(L);
@@ -359,7 +415,7 @@ DEV void do_op(Lane &L, u32 op, u64 trg, u64 &o1, u64 &o2) {
 }
 
 // instruction.rs:445-462
-DEV u64 do_op_num(Lane &L, u32 op, u64 trg, u32 rhs) {
+NOINL u64 do_op_num(Lane &L, u32 op, u64 trg, u32 rhs) {
   u64 port = ptag(trg) == T_RED ? resolve(L, trg) : trg;
   if (ptag(port) == T_NUM) {
     L.rw[RW_OPER]++;
@@ -379,7 +435,7 @@ DEV u64 do_op_num(Lane &L, u32 op, u64 trg, u32 rhs) {
 }
 
 // instruction.rs:465-493
-DEV void do_switch(Lane &L, u64 trg, u64 &o1, u64 &o2) {
+NOINL void do_switch(Lane &L, u64 trg, u64 &o1, u64 &o2) {
   u64 port = ptag(trg) == T_RED ? resolve(L, trg) : trg;
   if (ptag(port) == T_NUM) {
     L.rw[RW_OPER]++;
@@ -409,64 +465,88 @@ DEV void do_switch(Lane &L, u64 trg, u64 &o1, u64 &o2) {
   o2 = var_port(v2);
 }
 
-// def.rs:217-289
-DEV void interpret(Lane &L, u32 di, u64 t0) {
+// def.rs:217-289.  Instructions are fetched four at a time (one 64-byte
+// line); small definitions keep their trg registers in LDS.
+NOINL void exec_instr(Lane &L, u64 *T, u64 w0, u64 w1);
+NOINL void interpret(Lane &L, u32 di, u64 t0) {
   const Params &P = *L.P;
   const DefRec d = P.defs[di];
-  T_set(L, 0, t0);
+  u64 *T = (d.nslots <= LDS_SLOTS && L.T_lds) ? L.T_lds : L.T_glb;
+  if (d.nslots > P.trg_slots && T == L.T_glb) {
+    fail(L, ERR_BADPTR);
+    return;
+  }
+  T[0] = t0;
   const u64 *ip = P.instrs + (u64)d.ioff * 2;
-  for (u32 k = 0; k < d.ilen; k++) {
-    u64 w0 = ip[2 * k], w1 = ip[2 * k + 1];
+  for (u32 k = 0; k < d.ilen; k += 4) {
+    u64 w[8];
+#pragma unroll
+    for (u32 j = 0; j < 4; j++) {
+      bool in = k + j < d.ilen;
+      w[2 * j] = in ? ip[2 * (k + j)] : 0;
+      w[2 * j + 1] = in ? ip[2 * (k + j) + 1] : 0;
+    }
+    u32 cnt = d.ilen - k < 4 ? d.ilen - k : 4;
+    for (u32 j = 0; j < cnt; j++) exec_instr(L, T, w[2 * j], w[2 * j + 1]);
+  }
+}
+
+#define T_SET(s, v) (T[(u64)(s) * 64] = (v))
+NOINL void exec_instr(Lane &L, u64 *T, u64 w0, u64 w1) {
+  const Params &P = *L.P;
+  {
     u32 kind = (u32)(w0 & 0xFF);
     u32 a = (u32)(w0 >> 16) & 0xFFFF, b = (u32)(w0 >> 32) & 0xFFFF, c = (u32)(w0 >> 48);
     u32 d4 = kind == I_WIRES ? (u32)(w1 & 0xFFFF) : 0;
-    if (a >= P.trg_slots || b >= P.trg_slots || c >= P.trg_slots || d4 >= P.trg_slots) {
+    u32 lim = T == L.T_lds ? LDS_SLOTS : P.trg_slots;
+    if (a >= lim || b >= lim || c >= lim || d4 >= lim) {
       fail(L, ERR_BADPTR);
       return;
     }
     switch (kind) {
-      case I_CONST: T_set(L, a, w1); break;
-      case I_LINK: link(L, T_get(L, a), T_get(L, b)); break;
-      case I_LINKCONST: link(L, T_get(L, a), w1); break;
+      case I_CONST: T_SET(a, w1); break;
+      case I_LINK: link(L, T[(u64)a * 64], T[(u64)b * 64]); break;
+      case I_LINKCONST: link(L, T[(u64)a * 64], w1); break;
       case I_CTR: {
         u64 x, y;
-        do_ctr(L, (u32)(w1 & 0xFFFF), T_get(L, a), x, y);
-        T_set(L, b, x);
-        T_set(L, c, y);
+        do_ctr(L, (u32)(w1 & 0xFFFF), T[(u64)a * 64], x, y);
+        T_SET(b, x);
+        T_SET(c, y);
         break;
       }
       case I_OP: {
         u64 x, y;
-        do_op(L, (u32)(w1 & 0xFFFF), T_get(L, a), x, y);
-        T_set(L, b, x);
-        T_set(L, c, y);
+        do_op(L, (u32)(w1 & 0xFFFF), T[(u64)a * 64], x, y);
+        T_SET(b, x);
+        T_SET(c, y);
         break;
       }
-      case I_OPNUM: T_set(L, c, do_op_num(L, (u32)(w1 & 0xFFFF), T_get(L, a), (u32)(w1 >> 32))); break;
+      case I_OPNUM: T_SET(c, do_op_num(L, (u32)(w1 & 0xFFFF), T[(u64)a * 64], (u32)(w1 >> 32))); break;
       case I_SWITCH: {
         u64 x, y;
-        do_switch(L, T_get(L, a), x, y);
-        T_set(L, b, x);
-        T_set(L, c, y);
+        do_switch(L, T[(u64)a * 64], x, y);
+        T_SET(b, x);
+        T_SET(c, y);
         break;
       }
       case I_WIRES: {
         u64 v1 = alloc_var(L), v2 = alloc_var(L);
-        T_set(L, a, var_port(v1));
-        T_set(L, b, wire_port(v1));
-        T_set(L, c, var_port(v2));
-        T_set(L, (u32)(w1 & 0xFFFF), wire_port(v2));
+        T_SET(a, var_port(v1));
+        T_SET(b, wire_port(v1));
+        T_SET(c, var_port(v2));
+        T_SET((u32)(w1 & 0xFFFF), wire_port(v2));
         break;
       }
       default: fail(L, ERR_UNIMPL); return;
     }
   }
 }
+#undef T_SET
 
 DEV u64 expand_ref(u64 cell) { return mkport(T_REF, 0xFFFF, cell); }
 
 // net.rs:353-380.  `cell` holds the ExpandDef's `out` endpoint.
-DEV void expand_call(Lane &L, u64 cell, u64 port) {
+NOINL void expand_call(Lane &L, u64 cell, u64 port) {
   const Params &P = *L.P;
   if (ptag(port) == T_REF && port != P_ERA) {
     u64 oaddr = paddr(port);
@@ -501,12 +581,11 @@ DEV void expand_call(Lane &L, u64 cell, u64 port) {
   st(P, e2, var_port(v2));
   u64 n = new_node(L, t, plab(port), var_port(v1), var_port(v2));
   link(L, out, n);
-  link(L, p1, expand_ref(e1));
-  link(L, p2, expand_ref(e2));
+  link2(L, p1, expand_ref(e1), p2, expand_ref(e2));
 }
 
 // def.rs:177-189
-DEV void call(Lane &L, u64 port, u64 trg) {
+NOINL void call(Lane &L, u64 port, u64 trg) {
   const Params &P = *L.P;
   u64 addr = paddr(port);
   if (addr_in_heap(addr)) {  // ExpandDef: LabSet::ALL
@@ -533,7 +612,7 @@ DEV void call(Lane &L, u64 port, u64 trg) {
 }
 
 // interact.rs:6-44 (arm order preserved)
-DEV void interact(Lane &L, u64 a, u64 b) {
+NOINL void interact(Lane &L, u64 a, u64 b) {
   u32 at = ptag(a), bt = ptag(b);
   bool anil = at == T_REF || at == T_NUM, bnil = bt == T_REF || bt == T_NUM;
   if (at <= T_VAR || bt <= T_VAR) { fail(L, ERR_UNIMPL); return; }
@@ -563,16 +642,14 @@ DEV void interact(Lane &L, u64 a, u64 b) {
 // ---------------------------------------------------------------------------
 // lane setup / teardown
 // ---------------------------------------------------------------------------
-DEV void lane_init(Lane &L, const Params &P, u32 gwave, u32 lane) {
+DEV void lane_init(Lane &L, const Params &P, u32 gwave, u32 lane, u64 *alloc_ctr) {
   L.P = &P;
-  L.my_node0 = P.lane_node0 + (u64)gwave * P.nodes_per_lane * 64 + lane;
-  L.cursor = 0;
-  L.nslots = P.nodes_per_lane;
-  L.wrapped = false;
-  L.ov_mode = false;
-  L.ov_cur = L.ov_end = 0;
+  L.wave_node0 = P.ring_node0 + (u64)gwave * P.wave_nodes;
+  L.alloc_ctr = alloc_ctr;
   L.spare = 0;
-  L.T = P.trg_buf + (u64)gwave * P.trg_slots * 64 + lane;
+  L.dirty = false;
+  L.T_lds = nullptr;
+  L.T_glb = P.trg_buf + (u64)gwave * P.trg_slots * 64 + lane;
   L.bag = nullptr;
   L.bag_top = nullptr;
   L.bag_bot = 0;

@@ -31,7 +31,8 @@ EXPORTS = [
     "hvm_gpu_open", "hvm_gpu_close", "hvm_gpu_last_error", "hvm_gpu_load_book_text", "hvm_gpu_load_book",
     "hvm_gpu_def_id", "hvm_gpu_reset", "hvm_gpu_boot", "hvm_gpu_encode_net_text", "hvm_gpu_normal",
     "hvm_gpu_reduce", "hvm_gpu_rewrites", "hvm_gpu_shortcuts", "hvm_gpu_readback", "hvm_gpu_last_kernel_ms",
-    "hvm_gpu_grid", "hvm_gpu_set_timeout",
+    "hvm_gpu_grid", "hvm_gpu_set_timeout", "hvm_gpu_sched_stats",
+    "hvm_book_print", "hvm_book_label_sets", "hvm_net_apply_tree",
 ]
 
 _lib = None
@@ -73,6 +74,10 @@ def load_library(path=None):
     lib.hvm_gpu_last_kernel_ms.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
     lib.hvm_gpu_grid.argtypes = [P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
     lib.hvm_gpu_set_timeout.argtypes = [P, ctypes.c_double]
+    lib.hvm_gpu_sched_stats.argtypes = [P, ctypes.POINTER(u64)]
+    for name in ("hvm_book_print", "hvm_book_label_sets"):
+        getattr(lib, name).argtypes = [ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
+    lib.hvm_net_apply_tree.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
     _lib = lib
     return lib
 
@@ -233,10 +238,63 @@ class Net:
         self.heap._check(self.heap._lib.hvm_gpu_shortcuts(self.heap._h, out))
         return tuple(int(x) for x in out)
 
+    def sched_stats(self):
+        """Scheduler statistics of the last reduce launch (summed over waves)."""
+        out = (ctypes.c_uint64 * 7)()
+        self.heap._check(self.heap._lib.hvm_gpu_sched_stats(self.heap._h, out))
+        keys = ["busy_iters", "idle_iters", "busy_ticks", "spills", "spilled", "claims", "claimed"]
+        return dict(zip(keys, [int(x) for x in out]))
+
     def last_kernel_ms(self):
         ms = ctypes.c_double()
         self.heap._check(self.heap._lib.hvm_gpu_last_kernel_ms(self.heap._h, ctypes.byref(ms)))
         return ms.value
+
+
+def _host_text(fn, *args):
+    lib = load_library()
+    cap = 1 << 16
+    while True:
+        buf = ctypes.create_string_buffer(cap)
+        n = ctypes.c_uint64()
+        rc = fn(*[a.encode() for a in args], buf, cap, ctypes.byref(n))
+        if rc == 9 and n.value + 1 > cap:
+            cap = n.value + 1
+            continue
+        if rc != 0:
+            raise HvmError(rc, lib.hvm_gpu_last_error(None).decode())
+        return buf.value.decode()
+
+
+def book_print(book_text):
+    """Book::from_str + Display (ast/src/parser.rs:178-183, ast/src/ast.rs:455-466)."""
+    return _host_text(load_library().hvm_book_print, book_text)
+
+
+def label_sets(book_text):
+    """calculate_label_sets (host/src/calc_labels.rs:78-93) as {name: [labels]}."""
+    out = {}
+    for line in _host_text(load_library().hvm_book_label_sets, book_text).splitlines():
+        name, _, labs = line.partition(":")
+        out[name] = [int(x) for x in labs.split()]
+    return out
+
+
+def apply_tree(net_text, tree_text):
+    """Net::apply_tree (ast/src/ast.rs:346-357), as net text."""
+    return _host_text(load_library().hvm_net_apply_tree, net_text, tree_text)
+
+
+def book_apply(book_text, name, tree_text):
+    """Returns the book with definition `name` applied to `tree_text`
+    (the tests/lists.rs pattern: book.get_mut(name).apply_tree(tree))."""
+    printed = book_print(book_text)
+    defs = {}
+    for chunk in printed.split("\n\n"):
+        head, _, body = chunk.partition(" = ")
+        defs[head[1:]] = body
+    defs[name] = apply_tree(defs[name], tree_text)
+    return "\n\n".join(f"@{k} = {v}" for k, v in defs.items())
 
 
 def normal_with(book_text, entry="main", heap_bytes=1 << 30, heap=None):

@@ -1,0 +1,113 @@
+"""Seeded generators of the benchmark workloads, as .hvm book text.
+
+1. tree_sum(depth): the reference's tree-sum stress test
+   (tests/programs/stress_tests/sum_tree.hvm) at a chosen depth: `@gen`
+   builds a complete binary tree of CTR label-0 nodes with Num leaves 1 and
+   `@sum` folds it with u24 `+`.  Result 2**depth mod 2**24.
+2. bitonic(k, seed): the reference's bitonic sort
+   (examples/sort/bitonic/bitonic_sort_lam.hvm) over 2**k leaves whose keys
+   are a seeded u24 hash of the leaf position, computed in the net by `@hash`.
+   `@main` returns the u24 sum of the sorted keys (or the sorted tree).
+3. church(exp, labels): Church exponentiation c2 ^ c_exp applied to a Num
+   increment on 0, every numeral a pure CTR/DUP net; the duplicators of c_exp
+   cycle through `labels` distinct DUP labels.  Result 2**exp mod 2**24.
+"""
+import os
+import re
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_GOLDEN = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "tests", "golden", "programs")
+
+U24 = 1 << 24
+HASH_M1 = 0x5D2B93
+HASH_M2 = 0x3C6EF3
+
+
+def _fixture(rel):
+    with open(os.path.join(_GOLDEN, rel)) as f:
+        return f.read()
+
+
+def tree_sum(depth):
+    """Book whose @main sums a complete binary tree of 2**depth ones."""
+    text = _fixture("tests/programs/stress_tests/sum_tree.hvm")
+    text, n = re.subn(r"@gen ~ \(24 b\)", f"@gen ~ ({int(depth)} b)", text)
+    assert n == 1
+    return text
+
+
+def tree_sum_expected(depth):
+    return str((1 << depth) % U24)
+
+
+def key_hash(i, seed):
+    """u24 hash of a leaf position, as computed by the net's @hash."""
+    t = (i * HASH_M1 + seed) % U24
+    u = t ^ (t >> 11)
+    v = (u * HASH_M2) % U24
+    return v ^ (v >> 13)
+
+
+def _hash_def(seed):
+    return (f"@hash = (x k)\n"
+            f"& x ~ $([*{HASH_M1}] $([+{seed % U24}] {{t1 t2}}))\n"
+            f"& t1 ~ $([>>] $(11 s))\n"
+            f"& t2 ~ $([^] $(s u))\n"
+            f"& u ~ $([*{HASH_M2}] {{v1 v2}})\n"
+            f"& v1 ~ $([>>] $(13 w))\n"
+            f"& v2 ~ $([^] $(w k))\n")
+
+
+def bitonic(k, seed=1, result="sum"):
+    """Book whose @main bitonic-sorts 2**k hashed keys.
+
+    result="sum": the u24 sum of the sorted keys; result="tree": the sorted
+    Scott-encoded tree itself (for sortedness checks at small k)."""
+    text = _fixture("examples/sort/bitonic/bitonic_sort_lam.hvm")
+    text, n1 = re.subn(r"@gen\$S0 = \(a b\)\n& @Leaf ~ \(a b\)\n",
+                       "@gen$S0 = (a b)\n& @Leaf ~ (h b)\n& @hash ~ (a h)\n", text)
+    text, n2 = re.subn(r"& @gen ~ \(10 \(0 d\)\)", f"& @gen ~ ({int(k)} (0 d))", text)
+    assert n1 == 1 and n2 == 1
+    if result == "tree":
+        text, n3 = re.subn(r"@main = a\n& @sum ~ \(b a\)\n& @sort ~ \(c \(0 b\)\)",
+                           "@main = b\n& @sort ~ (c (0 b))", text)
+        assert n3 == 1
+    return text + _hash_def(seed)
+
+
+def bitonic_keys(k, seed=1):
+    return [key_hash(i, seed) for i in range(1 << k)]
+
+
+def bitonic_expected(k, seed=1):
+    return str(sum(bitonic_keys(k, seed)) % U24)
+
+
+def church(exp=22, labels=16, base=2):
+    """Book computing (c_base ^ c_exp) inc 0 with Church numerals.
+
+    c_n = ({d1 ... (x1 x2) ...} (x0 R)) (x_n R): f is copied n-1 times by a
+    chain of DUP nodes.  c_exp's duplicators use labels 2..labels+1 (cycling),
+    c_base's use label 1 (`{}`)."""
+    def numeral(n, lab_of):
+        # f applied n times: (f1 ... ) composition x0 -> x1 -> ... -> xn
+        if n == 1:
+            return "((a R) (a R))"
+        apps = [f"(x{i} x{i + 1})" for i in range(n)]
+        tree = apps[0]
+        for i in range(1, n):
+            lab = lab_of(i)
+            tree = f"#{lab}{{{tree} {apps[i]}}}"
+        return f"({tree} (x0 x{n}))"
+    cb = numeral(base, lambda i: 0)
+    cb = cb.replace("#0{", "{")
+    ce = numeral(exp, lambda i: 1 + (i - 1) % labels)
+    return (f"@base = {cb}\n"
+            f"@exp = {ce}\n"
+            f"@inc = ($([+1] r) r)\n"
+            f"@main = r\n"
+            f"& @exp ~ (@base (@inc (0 r)))\n")
+
+
+def church_expected(exp=22, base=2):
+    return str((base ** exp) % U24)

@@ -89,8 +89,9 @@ int hvm_gpu_load_book(hvm_gpu *h, const hvm_book_blob *book);
 /* ~ Host::defs[name] (host/src/host.rs:19): index of a loaded definition. */
 int hvm_gpu_def_id(hvm_gpu *h, const char *name, uint32_t *id);
 
-/* ~ Net::new(&heap) (runtime/src/net.rs:19-23): clears the heap, the redex
- * queue and the rewrite counters; the net's root is a fresh wire. */
+/* ~ Net::new(&heap) (runtime/src/net.rs:19-23): a fresh, empty net on the
+ * handle's heap: the allocation rings, the redex queue and the rewrite
+ * counters restart and the root is a fresh wire. */
 int hvm_gpu_reset(hvm_gpu *h);
 
 /* ~ Net::boot(&def) (runtime/src/net.rs:30-32): calls definition `def_id`
@@ -130,6 +131,27 @@ int hvm_gpu_last_kernel_ms(hvm_gpu *h, double *ms);
  * launch that runs past it stops every wave and returns HVM_E_TIMEOUT. */
 int hvm_gpu_set_timeout(hvm_gpu *h, double seconds);
 int hvm_gpu_grid(hvm_gpu *h, uint32_t *blocks, uint32_t *waves);
+
+/* Scheduler statistics of the last reduction launch, summed over waves:
+ * iterations with work, idle iterations, 100 MHz ticks spent in iterations
+ * with work, spills to the global ring, redexes spilled, claims from the
+ * global ring, redexes claimed. */
+int hvm_gpu_sched_stats(hvm_gpu *h, uint64_t out[7]);
+
+/* Host-only book utilities (no device).  Text results follow
+ * hvm_gpu_readback's buffer convention; errors are described by
+ * hvm_gpu_last_error(NULL). */
+
+/* ~ Book::from_str + Display (ast/src/parser.rs:178-183, ast/src/ast.rs:455-466). */
+int hvm_book_print(const char *book_text, char *buf, uint64_t cap, uint64_t *len);
+
+/* ~ calculate_label_sets (host/src/calc_labels.rs:78-93): one line per
+ * definition, "name: l1 l2 ...". */
+int hvm_book_label_sets(const char *book_text, char *buf, uint64_t cap, uint64_t *len);
+
+/* ~ Net::apply_tree (ast/src/ast.rs:346-357): `net_text` applied to
+ * `tree_text`, printed as a net. */
+int hvm_net_apply_tree(const char *net_text, const char *tree_text, char *buf, uint64_t cap, uint64_t *len);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,129 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure only: the oracle is the checker, never the thing
+measured or shipped."""
+import ctypes
+import os
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Oracle:
+    def __init__(self, path=LIB):
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run `make -C oracle`")
+        self.lib = lib = ctypes.CDLL(path)
+        lib.oracle_last_error.restype = ctypes.c_char_p
+        u64 = ctypes.c_uint64
+        lib.oracle_run.argtypes = [ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                   u64, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                   ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double)]
+        lib.oracle_reduce_expr.argtypes = [ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.c_char_p, u64,
+                                           ctypes.POINTER(u64), ctypes.POINTER(u64)]
+        lib.oracle_apply_tree.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, u64,
+                                          ctypes.c_char_p, u64, ctypes.POINTER(u64)]
+        lib.oracle_print_book.argtypes = [ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
+        lib.oracle_label_sets.argtypes = [ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
+        lib.oracle_operate_binary.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        lib.oracle_operate_binary.restype = ctypes.c_uint32
+        lib.oracle_operate_unary.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        lib.oracle_operate_unary.restype = ctypes.c_uint32
+        lib.oracle_new_f24.argtypes = [ctypes.c_float]
+        lib.oracle_new_f24.restype = ctypes.c_uint32
+        lib.oracle_num_display.argtypes = [ctypes.c_uint32, ctypes.c_char_p, u64]
+        lib.oracle_parse_num.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32)]
+        lib.oracle_create_var.argtypes = [u64, ctypes.c_char_p, u64]
+
+    def _err(self, rc):
+        raise OracleError(f"oracle error {rc}: {self.lib.oracle_last_error().decode()}")
+
+    def run(self, book, entry="main", heap_words=1 << 27, threads=0, meter=False):
+        """Host::new + boot(entry) + normal() (threads=0) or parallel_normal().
+
+        Returns dict(nf, rwts, shortcuts, peak_nodes, meter, seconds)."""
+        cap = 1 << 20
+        while True:
+            buf = ctypes.create_string_buffer(cap)
+            n = ctypes.c_uint64()
+            c = (ctypes.c_uint64 * 5)()
+            e = (ctypes.c_uint64 * 4)()
+            m = (ctypes.c_uint64 * 3)()
+            s = ctypes.c_double()
+            rc = self.lib.oracle_run(book.encode(), entry.encode(), heap_words, threads, int(meter), buf, cap,
+                                     ctypes.byref(n), c, e, m, ctypes.byref(s))
+            if rc == 7 and n.value + 1 > cap:
+                cap = n.value + 1
+                continue
+            if rc:
+                self._err(rc)
+            rw = dict(anni=c[0], comm=c[1], eras=c[2], dref=c[3], oper=c[4])
+            rw["rwts"] = sum(c)
+            return dict(nf=buf.value.decode(), rwts=rw, shortcuts=tuple(e[:3]), peak_nodes=e[3],
+                        meter=dict(loads=m[0], stores=m[1], rmws=m[2]), seconds=s.value)
+
+    def reduce_expr(self, book, expr, heap_words=1 << 24):
+        buf = ctypes.create_string_buffer(1 << 20)
+        n = ctypes.c_uint64()
+        c = (ctypes.c_uint64 * 5)()
+        rc = self.lib.oracle_reduce_expr(book.encode(), expr.encode(), heap_words, buf, 1 << 20, ctypes.byref(n), c)
+        if rc:
+            self._err(rc)
+        return buf.value.decode(), list(c)
+
+    def apply_tree_normal(self, fun, args, heap_words=16):
+        arr = (ctypes.c_char_p * max(len(args), 1))(*[a.encode() for a in args])
+        buf = ctypes.create_string_buffer(1 << 16)
+        n = ctypes.c_uint64()
+        rc = self.lib.oracle_apply_tree(fun.encode(), arr, len(args), heap_words, buf, 1 << 16, ctypes.byref(n))
+        if rc:
+            self._err(rc)
+        return buf.value.decode()
+
+    def _text(self, fn, *args):
+        buf = ctypes.create_string_buffer(1 << 22)
+        n = ctypes.c_uint64()
+        rc = fn(*[a.encode() for a in args], buf, 1 << 22, ctypes.byref(n))
+        if rc:
+            self._err(rc)
+        return buf.value.decode()
+
+    def print_book(self, book):
+        return self._text(self.lib.oracle_print_book, book)
+
+    def label_sets(self, book):
+        out = {}
+        for line in self._text(self.lib.oracle_label_sets, book).splitlines():
+            name, _, labs = line.partition(":")
+            out[name] = [int(x) for x in labs.split()]
+        return out
+
+    def operate_binary(self, a, op, b):
+        return self.lib.oracle_operate_binary(a, op, b)
+
+    def operate_unary(self, op, a):
+        return self.lib.oracle_operate_unary(op, a)
+
+    def new_f24(self, f):
+        return self.lib.oracle_new_f24(f)
+
+    def num_display(self, raw):
+        buf = ctypes.create_string_buffer(64)
+        self.lib.oracle_num_display(raw, buf, 64)
+        return buf.value.decode()
+
+    def parse_num(self, text):
+        raw = ctypes.c_uint32()
+        rc = self.lib.oracle_parse_num(text.encode(), ctypes.byref(raw))
+        if rc:
+            self._err(rc)
+        return raw.value
+
+    def create_var(self, i):
+        buf = ctypes.create_string_buffer(64)
+        self.lib.oracle_create_var(i, buf, 64)
+        return buf.value.decode()

@@ -1,0 +1,205 @@
+"""GPU parity: the HIP engine against the reference's golden snapshots and the
+oracle.  Normal forms must be identical text and all five rewrite counts
+identical (integer work: bit-exact)."""
+import os
+import random
+import re
+
+import pytest
+
+from conftest import GOLDEN, load_snapshots
+
+pytestmark = pytest.mark.gpu
+
+SNAPS = load_snapshots()
+# Programs whose reduction is parallel enough to finish in seconds on the GPU;
+# the tail-recursive stress tests (sum_tail, tuple_rots, boom, ...) are long
+# sequential chains and are exercised by the oracle tests instead.
+GPU_CASES = sorted((k for k, v in SNAPS.items() if v["rwts"]["rwts"] <= 3_000_000), key=lambda k: SNAPS[k]["rwts"]["rwts"])
+GPU_CASES += ["tests::programs::stress_tests::fib_rec", "tests::programs::stress_tests::sum_rec"]
+
+
+def run_gpu(heap, text, entry="main"):
+    import hvm64
+    host = hvm64.Host(text, heap)
+    net = hvm64.Net(heap)
+    net.boot(host.defs[entry])
+    net.normal()
+    return host.readback(net), net.rwts.as_dict(), net
+
+
+@pytest.mark.parametrize("name", GPU_CASES)
+def test_snapshot(gpu_heap, name):
+    snap = SNAPS[name]
+    print("case", name, flush=True)
+    nf, rw, _ = run_gpu(gpu_heap, snap["text"])
+    assert nf == snap["normal_form"]
+    assert rw == snap["rwts"]
+
+
+@pytest.mark.parametrize("cfg,size", [("tree_sum", 10), ("tree_sum", 14), ("tree_sum", 18),
+                                      ("bitonic", 6), ("bitonic", 9), ("bitonic", 12),
+                                      ("church", 4), ("church", 10), ("church", 14)])
+def test_config_against_oracle(gpu_heap, oracle, cfg, size):
+    from hvm64 import configs
+    print("case", cfg, size, flush=True)
+    text = {"tree_sum": lambda: configs.tree_sum(size), "bitonic": lambda: configs.bitonic(size, 7),
+            "church": lambda: configs.church(size, 16)}[cfg]()
+    expected = {"tree_sum": lambda: configs.tree_sum_expected(size), "bitonic": lambda: configs.bitonic_expected(size, 7),
+                "church": lambda: configs.church_expected(size)}[cfg]()
+    nf, rw, net = run_gpu(gpu_heap, text)
+    ref = oracle.run(text, heap_words=1 << 28)
+    assert nf == expected == ref["nf"]
+    assert rw == ref["rwts"]
+    assert net.shortcuts == (0, 0, 0)
+
+
+@pytest.mark.parametrize("k,seed", [(5, 3), (10, 2024)])
+def test_bitonic_tree_sorted(gpu_heap, oracle, k, seed):
+    from hvm64 import configs
+    text = configs.bitonic(k, seed, result="tree")
+    nf, rw, _ = run_gpu(gpu_heap, text)
+    keys = [int(x) for x in re.findall(r"\(\((\d+) [a-z]+\)", nf)]
+    assert keys == sorted(configs.bitonic_keys(k, seed))
+    ref = oracle.run(text, heap_words=1 << 26)
+    assert nf == ref["nf"] and rw == ref["rwts"]
+
+
+APPLY_CASES = [
+    ("(a a)", ["(a a)"], "(a a)"),
+    ("b & (a b) ~ a", ["(a a)"], "a"),
+    ("(z0 z0)", ["(z1 z1)"], "(a a)"),
+    ("(* 1)", ["(a a)"], "1"),
+    ("($([+] $(a b)) (a b))", ["1", "2"], "3"),
+    ("($([*] $(a b)) (a b))", ["2", "3"], "6"),
+    ("($([*] $(a b)) (a b))", ["2"], "($([*2] a) a)"),
+]
+
+
+@pytest.mark.parametrize("fun,args,expected", APPLY_CASES)
+def test_apply_tree_encode_net(gpu_heap, fun, args, expected):
+    """tests/cli.rs:206-251 through Host::encode_net + normal() on the GPU."""
+    import hvm64
+    net_text = fun
+    for a in args:
+        net_text = hvm64.apply_tree(net_text, a)
+    host = hvm64.Host("", gpu_heap)
+    net = hvm64.Net(gpu_heap)
+    host.encode_net(net, net_text)
+    net.normal()
+    assert host.readback(net) == expected
+
+
+ARITH = open(os.path.join(GOLDEN, "programs/examples/arithmetic.hvm")).read()
+
+
+@pytest.mark.parametrize("book,expr,expected", [
+    ("", "1", "1"), ("", "a & 3 ~ $([*] $(4 a))", "12"), (ARITH, "a & @mul ~ (3 (4 a))", "12"),
+    (ARITH, "a & @div ~ (64 (2 a))", "32"),
+    (ARITH, "a\n  & @main ~ (64 a)", "(#2{$([/64] a) $([%64] b)} {a b})"),
+])
+def test_reduce_expressions(gpu_heap, oracle, book, expr, expected):
+    """tests/cli.rs:41-131 (`hvm64 reduce` / `run`): encode_net, normal, readback."""
+    import hvm64
+    host = hvm64.Host(book, gpu_heap)
+    net = hvm64.Net(gpu_heap)
+    host.encode_net(net, expr)
+    net.normal()
+    nf = host.readback(net)
+    ref_nf, ref_counts = oracle.reduce_expr(book, expr)
+    assert nf == expected == ref_nf
+    r = net.rwts
+    assert [r.anni, r.comm, r.eras, r.dref, r.oper] == ref_counts
+
+
+@pytest.mark.parametrize("kind,cases,expected", [
+    ("got", [(0,), (3,), (31,)], [594, 681, 1493]),
+    ("put", [(0, 2), (7, 16), (31, 0)], [585, 795, 1515]),
+])
+def test_lists_counts(gpu_heap, kind, cases, expected):
+    """tests/lists.rs:29-77 count vectors."""
+    import hvm64
+    text = open(os.path.join(GOLDEN, "programs/tests/programs/list_put_got.hvm")).read()
+    totals = []
+    for args in cases:
+        if kind == "got":
+            book = hvm64.book_apply(text, "GenGotIndex", f"@S{args[0]}")
+            book = hvm64.book_apply(book, "main", "@GenGotIndex")
+        else:
+            book = hvm64.book_apply(text, "GenPutIndexValue", f"@S{args[0]}")
+            book = hvm64.book_apply(book, "GenPutIndexValue", f"@S{args[1]}")
+            book = hvm64.book_apply(book, "main", "@GenPutIndexValue")
+        totals.append(run_gpu(gpu_heap, book)[1]["rwts"])
+    assert totals == expected
+
+
+def numeric_book(seed, n=120):
+    """A chain of random binary operations on random u24 / i24 / f24 operands,
+    in the style of tests/programs/u24.hvm."""
+    rng = random.Random(seed)
+    ops = ["+", "-", ":-", "*", "/", ":/", "%", ":%", "&", "|", "^", "<<", ":<<", ">>", ":>>", "=", "!", "<", ">"]
+    lines = ["@main = x & @t0 ~ (* x)"]
+    for i in range(n):
+        ty = rng.choice(["u", "i", "f"])
+        op = rng.choice(ops)
+        if ty == "u":
+            a, b = rng.randrange(0, 1 << 24), rng.randrange(1, 1 << 24)
+            if op in (":/", ":%") and a == 0:
+                a = 1
+            a, b = str(a), str(b)
+        elif ty == "i":
+            a, b = rng.randrange(-(1 << 23), 1 << 23), rng.randrange(-(1 << 23), 1 << 23)
+            b = b or 1
+            a = a or 1
+            a, b = f"{a:+d}", f"{b:+d}"
+        else:
+            if op in ("&", "|", "^", "<<", ":<<", ">>", ":>>"):
+                op = "+"
+            a, b = f"{rng.uniform(-1e4, 1e4):.3f}", f"{rng.uniform(-1e4, 1e4):.3f}"
+        lines.append(f"@t{i} = (* {{n x}}) & @t{i + 1} ~ (* x) & {a} ~ $([{op}] $({b} n))")
+    lines.append(f"@t{n} = *")
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_numeric_ops_against_oracle(gpu_heap, oracle, seed):
+    """u24 / i24 / f24 operators on the device give bit-identical results."""
+    text = numeric_book(seed)
+    nf, rw, _ = run_gpu(gpu_heap, text)
+    ref = oracle.run(text, heap_words=1 << 22)
+    assert nf == ref["nf"]
+    assert rw == ref["rwts"]
+
+
+def test_reduce_limit_then_normal(gpu_heap, oracle):
+    """Net::reduce(limit) (net.rs:41-52) stops with redexes left (None) and a
+    later normal() reaches the same normal form with the same counts."""
+    import hvm64
+    from hvm64 import configs
+    text = configs.bitonic(8, 5)
+    host = hvm64.Host(text, gpu_heap)
+    net = hvm64.Net(gpu_heap)
+    net.boot(host.defs["main"])
+    assert net.reduce(1000) is None
+    popped = net.rwts.total()
+    assert popped > 0
+    net.normal()
+    ref = oracle.run(text, heap_words=1 << 24)
+    assert host.readback(net) == ref["nf"]
+    assert net.rwts.as_dict() == ref["rwts"]
+
+
+@pytest.mark.parametrize("text,kind", [
+    ("@main = a & 1 ~ $([/] $(0 a))", "DIV0"),
+    ("@main = a & +5 ~ $([%] $(+0 a))", "DIV0"),
+    ("@main = * & $(a b) ~ ?(a b)", "UNIMPL_PAIR"),
+])
+def test_reference_panics_become_errors(gpu_heap, text, kind):
+    """Rust panics of the reference (num.rs:183-212, interact.rs:39-42) are
+    reported as error codes; the handle stays usable."""
+    import hvm64
+    with pytest.raises(hvm64.HvmError) as e:
+        run_gpu(gpu_heap, text)
+    assert e.value.kind == kind
+    nf, rw, _ = run_gpu(gpu_heap, SNAPS["examples::machine_u32::num_add"]["text"])
+    assert nf == "223"
