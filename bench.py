@@ -92,6 +92,9 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--heap-gb", type=float, default=32.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="before the steps, copy a buffer of this many MiB on the device (a known-size "
+                         "kernel that calibrates HBM byte counters in a profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs one net per "
@@ -110,6 +113,14 @@ def main():
         net.boot(main_def)
         net.normal()
         return net
+
+    if args.copy_mib:
+        import torch
+        src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     for _ in range(args.warmup):
         net = step()

@@ -155,7 +155,7 @@ void reset_net(hvm_gpu *h, bool fill) {
   if (fill) hip_check(launch_fill(h->heap, h->heap_words, P_FREE, h->stream), "fill heap");
   hip_check(launch_fill(h->queue, h->qslots * 2, 0, h->stream), "clear ring");
   Globals g0{};
-  hip_check(hipMemsetAsync(h->alloc_pos, 0, (size_t)h->waves * sizeof(u64), h->stream), "clear rings");
+  hip_check(hipMemsetAsync(h->alloc_pos, 0, (size_t)h->waves * 64 * sizeof(u64), h->stream), "clear rings");
   hip_check(hipMemcpyAsync(h->g, &g0, sizeof(Globals), hipMemcpyHostToDevice, h->stream), "init globals");
   u64 none = P_NONE;
   hip_check(hipMemcpyAsync(h->heap + ROOT0_WORD, &none, sizeof(u64), hipMemcpyHostToDevice, h->stream), "root");
@@ -261,21 +261,27 @@ int hvm_gpu_open(int device, uint64_t heap_bytes, hvm_gpu **out) {
     h->heap_words = words;
     hip_check(hipMalloc(&h->heap, words * 8), "hipMalloc heap");
     // Node layout: reserved words, one ring of 2^k nodes per wave (k as large
-    // as fits in 7/8 of the heap), then the shared overflow ring (2^j nodes).
+    // as fits in 7/8 of the heap; each lane allocates every 64th node of its
+    // wave's ring), then the shared overflow ring (2^j nodes).
     u64 nodes = (words - RESERVED_WORDS) / 2;
     u64 per_wave = 1;
     while (per_wave * 2 * h->waves <= nodes / 8 * 7) per_wave *= 2;
+    if (per_wave < 64) throw std::runtime_error("heap too small for the grid");
     h->ring_node0 = RESERVED_WORDS / 2;
     h->wave_nodes = per_wave;
     u64 rest = nodes - per_wave * h->waves;
     u64 ov = 1;
     while (ov * 2 <= rest) ov *= 2;
     h->ov_node0 = h->ring_node0 + per_wave * h->waves;
+    if (ov < 64) throw std::runtime_error("heap too small for the grid");
     h->ov_nodes = ov;
-    hip_check(hipMalloc(&h->alloc_pos, (size_t)h->waves * sizeof(u64)), "hipMalloc rings");
+    hip_check(hipMalloc(&h->alloc_pos, (size_t)h->waves * 64 * sizeof(u64)), "hipMalloc rings");
     hip_check(hipMalloc(&h->g, sizeof(Globals)), "hipMalloc globals");
-    h->qslots = 1ull << 22;
-    while (h->qslots < (u64)h->waves * 512) h->qslots <<= 1;
+    // Global redex ring: 2^26 slots (1 GiB).  A massively parallel net keeps
+    // about (lanes x recursion depth) redexes pending; the LDS rings hold
+    // 256 per wave and the rest waits here.
+    h->qslots = 1ull << 26;
+    while (h->qslots < (u64)h->waves * 1024) h->qslots <<= 1;
     hip_check(hipMalloc(&h->queue, h->qslots * 2 * sizeof(u64)), "hipMalloc ring");
     int wall_khz = 0;
     hip_check(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device), "wall clock rate");
@@ -558,9 +564,13 @@ int hvm_net_apply_tree(const char *net_text, const char *tree_text, char *buf, u
       buf, cap, len);
 }
 
-int hvm_gpu_sched_stats(hvm_gpu *h, uint64_t out[7]) {
+int hvm_gpu_sched_stats(hvm_gpu *h, uint64_t out[20]) {
   return guarded(h, [&] {
-    for (u32 k = 0; k < ST_SLOTS; k++) out[k] = h->last.st[k];
+    for (u32 k = 0; k < ST_CLS_IT; k++) out[k] = h->last.st[k];
+    out[9] = h->last.rw[RW_ALLOC_MISS];
+    out[10] = h->last.rw[RW_OV_ALLOC];
+    for (u32 k = 0; k < 8; k++) out[11 + k] = h->last.st[ST_CLS_IT + k];
+    out[19] = 0;
   });
 }
 

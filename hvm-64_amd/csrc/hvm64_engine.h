@@ -31,12 +31,24 @@ struct DefRec {
 // Rewrite counter slots (runtime.rs:72-79) and interpreter shortcut counters
 // (ERA targets of do_op / do_op_num / do_switch, instruction.rs:144-197),
 // and the number of redexes popped.
-enum : u32 { RW_ANNI = 0, RW_COMM, RW_ERAS, RW_DREF, RW_OPER, SC_OP_ERA, SC_OPNUM_ERA, SC_SWITCH_ERA, RW_POPS, RW_SLOTS };
+enum : u32 {
+  RW_ANNI = 0, RW_COMM, RW_ERAS, RW_DREF, RW_OPER, SC_OP_ERA, SC_OPNUM_ERA, SC_SWITCH_ERA, RW_POPS,
+  RW_ALLOC_MISS,  // ring slots found in use on a later lap
+  RW_OV_ALLOC,    // allocations served by the overflow ring
+  RW_SLOTS
+};
 
 // Scheduler statistics (summed over waves): iterations with work, idle
 // iterations, wall-clock ticks (100 MHz) spent in iterations with work,
 // spills to the global ring / redexes spilled, claims / redexes claimed.
-enum : u32 { ST_BUSY_IT = 0, ST_IDLE_IT, ST_BUSY_TICKS, ST_SPILLS, ST_SPILLED, ST_CLAIMS, ST_CLAIMED, ST_SLOTS };
+enum : u32 {
+  ST_BUSY_IT = 0, ST_IDLE_IT, ST_BUSY_TICKS, ST_SPILLS, ST_SPILLED, ST_CLAIMS, ST_CLAIMED,
+  ST_INTERACT_TICKS,  // ticks inside interact() (the divergent section)
+  ST_SPILL_TICKS,     // ticks spilling to the global ring
+  ST_CLS_IT,          // 4 slots: iterations that ran each redex class
+  ST_CLS_TICKS = ST_CLS_IT + 4,  // 4 slots: ticks inside the rules, per class
+  ST_SLOTS = ST_CLS_TICKS + 4
+};
 
 // Error codes reported by the device (mirrored in the C ABI header).
 enum : u32 {

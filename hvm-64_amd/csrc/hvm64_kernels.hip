@@ -50,7 +50,7 @@ __global__ void k_fill(u64 *words, u64 n, u64 value) {
 __global__ void k_exec(Params P, u32 def, u64 t0, u32 count_dref) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   Lane L;
-  lane_init(L, P, 0, 0, &P.alloc_pos[0]);
+  lane_init(L, P, 0, 0);
   if (count_dref) L.rw[RW_DREF]++;
   interpret(L, def, t0);
   publish();
@@ -65,7 +65,7 @@ __global__ void k_exec(Params P, u32 def, u64 t0, u32 count_dref) {
 __global__ void k_expand(Params P, u64 old_root_cell) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   Lane L;
-  lane_init(L, P, 0, 0, &P.alloc_pos[0]);
+  lane_init(L, P, 0, 0);
   u64 r2 = alloc_var(L);
   u64 e = alloc_var(L);
   st(P, e, var_port(r2));
@@ -77,12 +77,14 @@ __global__ void k_expand(Params P, u64 old_root_cell) {
     if (L.rw[k]) __hip_atomic_fetch_add(&P.g->rw[k], (u64)L.rw[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// The persistent reduction kernel (net.rs:41-60).  Every wave owns an LDS
-// ring of redexes; each iteration its 64 lanes pop up to 64 redexes (LIFO)
-// and interact them; new redexes are pushed back into the ring.  A wave that
-// ends an iteration with more than one iteration's worth of redexes spills
-// the excess (its oldest redexes) to the global ring while other waves are
-// idle; an empty wave claims up to 64 redexes from the global ring.
+// The persistent reduction kernel (net.rs:41-60).  Every wave keeps its
+// redexes in LDS rings, one per rule class (redex_class); each iteration it
+// picks the fullest class and its 64 lanes pop up to 64 redexes of it (LIFO)
+// and interact them, so the lanes of an iteration run the same rule.  New
+// redexes are pushed into the rings by class.  A wave that ends an iteration
+// with more than one iteration's worth of redexes spills the excess (the
+// oldest redexes of its fullest class) to the global ring while other waves
+// are idle; an empty wave claims up to 64 redexes from the global ring.
 // S = (queued << 32) | active waves; the reduction is complete when S == 0,
 // a state no wave can leave.  S is one cache line shared by every wave, so
 // busy waves touch it only to spill (and read it every 16 iterations), and
@@ -94,27 +96,53 @@ DEV void wave_sleep(u32 units) {
   for (u32 z = 0; z < units; z++) __builtin_amdgcn_s_sleep(2);
 }
 
+DEV u32 uniform(u32 v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Moves `cnt` redexes, the oldest of class `c`, from the wave's rings to the
+// global ring (all lanes; lane 0 updates the counters).  Returns S after.
+DEV u64 spill_to_global(const Params &P, u64 *bag, u32 *bot_c, u32 c, u32 cnt, u32 lane, u32 bot, u32 &err) {
+  u64 base = 0;
+  if (lane == 0) base = __hip_atomic_fetch_add(&P.g->q_tail, (u64)cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  base = __shfl(base, 0, 64);
+  for (u32 i = lane; i < cnt; i += 64) {
+    u32 s2 = (c * CLS_CAP + ((bot + i) & (CLS_CAP - 1))) * 2;
+    u64 *q = P.queue + ((base + i) & P.qmask) * 2;
+    gst(q, bag[s2]);
+    gst(q + 1, bag[s2 + 1]);
+  }
+  publish();
+  u64 old = 0;
+  if (lane == 0) {
+    old = __hip_atomic_fetch_add(&P.g->S, (u64)cnt << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((old >> 32) + cnt > (P.qmask + 1) / 2) err = ERR_QUEUE;
+    *bot_c = bot + cnt;
+  }
+  return __shfl(old, 0, 64) + ((u64)cnt << 32);
+}
+
 __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
-  __shared__ u64 s_bag[WAVES_PER_BLOCK][BAG_CAP * 2];
-  __shared__ u64 s_T[WAVES_PER_BLOCK][LDS_SLOTS * 64];
-  __shared__ u64 s_alloc[WAVES_PER_BLOCK];
-  __shared__ u32 s_top[WAVES_PER_BLOCK];
-  __shared__ u32 s_bot[WAVES_PER_BLOCK];
+  __shared__ u64 s_bag[WAVES_PER_BLOCK][NCLS * CLS_CAP * 2];
+  __shared__ u64 s_T[WAVES_PER_BLOCK][(LDS_SLOTS + 2 * LINK_CAP) * 64];
+  __shared__ u32 s_top[WAVES_PER_BLOCK][NCLS];
+  __shared__ u32 s_bot[WAVES_PER_BLOCK][NCLS];
   const u32 wib = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const u32 gwave = blockIdx.x * WAVES_PER_BLOCK + wib;
   Globals *g = P.g;
   u64 *bag = s_bag[wib];
-  if (lane == 0) {
-    s_top[wib] = 0;
-    s_bot[wib] = 0;
-    s_alloc[wib] = P.alloc_pos[gwave];
+  u32 *top_c = s_top[wib];
+  u32 *bot_c = s_bot[wib];
+  if (lane < NCLS) {
+    top_c[lane] = 0;
+    bot_c[lane] = 0;
   }
   __builtin_amdgcn_wave_barrier();
   Lane L;
-  lane_init(L, P, gwave, lane, &s_alloc[wib]);
+  lane_init(L, P, gwave, lane);
   L.T_lds = &s_T[wib][lane];
+  L.PL = &s_T[wib][LDS_SLOTS * 64 + lane];
   L.bag = bag;
-  L.bag_top = &s_top[wib];
+  L.bag_top = top_c;
+  L.bag_bot = bot_c;
   __builtin_amdgcn_wave_barrier();
   bool active = false;
   const bool limited = P.limit != ~0ull;
@@ -125,7 +153,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   u64 S_next = 0;              // lane 0: control words read at iteration end
   u32 err_next = 0;
   bool fresh = false;          // S_next / err_next hold a read to consume
-  u64 st[ST_SLOTS] = {0, 0, 0, 0, 0, 0, 0};
+  u64 st[ST_SLOTS] = {};
 
   for (;;) {
     if (fresh) {
@@ -138,10 +166,15 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
         break;
       }
     }
-    u32 top = s_top[wib], bot = s_bot[wib];
-    u32 n = top - bot;
+    // ring occupancy: lane c < NCLS reads class c
+    u32 my_top = lane < NCLS ? top_c[lane] : 0, my_bot = lane < NCLS ? bot_c[lane] : 0;
+    u32 my_n = my_top - my_bot;
+    u32 n = my_n;
+#pragma unroll
+    for (u32 o = 1; o < NCLS; o <<= 1) n += __shfl_xor(n, o, 64);
+    n = uniform(n);
 
-    // ---- empty ring: claim work from the global ring, or idle
+    // ---- empty rings: claim work from the global ring, or idle
     if (n == 0) {
       u32 k = 0;
       u64 s = 0;
@@ -208,18 +241,32 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
         }
         gst(slot, 0);
         gst(slot + 1, 0);
-        u32 i = ((bot + lane) & (BAG_CAP - 1)) * 2;
+        // the rings are empty, so k <= 64 entries always fit
+        u32 c = redex_class(a, b);
+        u32 idx = HVM_LDS_FAA(&top_c[c], 1u);
+        u32 i = (c * CLS_CAP + (idx & (CLS_CAP - 1))) * 2;
         bag[i] = a;
         bag[i + 1] = b;
       }
-      if (lane == 0) s_top[wib] = bot + k;
       __builtin_amdgcn_wave_barrier();
-      top = bot + k;
-      n = k;
+      continue;
     }
 
+    // ---- choose the fullest class (lowest index on ties)
+    u32 key = lane < NCLS ? (my_n << 2) | (NCLS - 1 - lane) : 0;
+#pragma unroll
+    for (u32 o = 1; o < NCLS; o <<= 1) {
+      u32 other = __shfl_xor(key, o, 64);
+      key = other > key ? other : key;
+    }
+    key = uniform(key);
+    const u32 c = NCLS - 1 - (key & 3);
+    const u32 nc = key >> 2;
+    const u32 top = uniform(__shfl(my_top, c, 64));
+    const u32 bot = uniform(__shfl(my_bot, c, 64));
+
     // ---- budget for reduce(limit) (net.rs:41-52)
-    u32 take = n < 64 ? n : 64;
+    u32 take = nc < 64 ? nc : 64;
     if (limited) {
       u32 granted = 0;
       if (lane == 0) {
@@ -237,23 +284,16 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
           }
         }
       }
-      take = __shfl(granted, 0, 64);
+      take = uniform(__shfl(granted, 0, 64));
       if (take == 0) {
-        // budget spent: hand the whole ring back to the global queue and stop
-        u64 base = 0;
-        if (lane == 0) base = __hip_atomic_fetch_add(&g->q_tail, (u64)n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        base = __shfl(base, 0, 64);
-        for (u32 i = lane; i < n; i += 64) {
-          u32 s2 = ((bot + i) & (BAG_CAP - 1)) * 2;
-          u64 *q = P.queue + ((base + i) & P.qmask) * 2;
-          gst(q, bag[s2]);
-          gst(q + 1, bag[s2 + 1]);
+        // budget spent: hand every ring back to the global queue and stop
+        u32 qerr = 0;
+        for (u32 cc = 0; cc < NCLS; cc++) {
+          u32 t2 = uniform(__shfl(my_top, cc, 64)), b2 = uniform(__shfl(my_bot, cc, 64));
+          if (t2 != b2) S_cache = spill_to_global(P, bag, &bot_c[cc], cc, t2 - b2, lane, b2, qerr);
         }
-        publish();
-        if (lane == 0) {
-          s_bot[wib] = top;
-          __hip_atomic_fetch_add(&g->S, ((u64)n << 32) - (active ? 1 : 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        if (qerr) fail(L, qerr);
+        if (lane == 0 && active) __hip_atomic_fetch_sub(&g->S, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         active = false;
         break;
       }
@@ -265,17 +305,28 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     bool has = lane < take;
     u64 ra = 0, rb = 0;
     if (has) {
-      u32 i = ((top - 1 - lane) & (BAG_CAP - 1)) * 2;
+      u32 i = (c * CLS_CAP + ((top - 1 - lane) & (CLS_CAP - 1))) * 2;
       ra = bag[i];
       rb = bag[i + 1];
     }
     __builtin_amdgcn_wave_barrier();
-    if (lane == 0) s_top[wib] = top - take;
+    if (lane == 0) top_c[c] = top - take;
     __builtin_amdgcn_wave_barrier();
-    L.bag_bot = bot;
+    const u64 t_in = wall_clock64();
     if (has) {
       L.rw[RW_POPS]++;
       interact(L, ra, rb);
+    }
+    {
+      const u64 dt = wall_clock64() - t_in;
+      st[ST_INTERACT_TICKS] += dt;
+      // c is wave-uniform; unrolled so st stays in registers
+#pragma unroll
+      for (u32 cc = 0; cc < NCLS; cc++)
+        if (c == cc) {
+          st[ST_CLS_IT + cc]++;
+          st[ST_CLS_TICKS + cc] += dt;
+        }
     }
     if (++busy >= REFRESH) {  // in flight while this iteration's stores retire
       busy = 0;
@@ -289,40 +340,41 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     L.dirty = false;
     __builtin_amdgcn_wave_barrier();
 
-    // ---- clamp the ring after overflow pushes, then spill the excess
-    top = s_top[wib];
-    if (top - bot > BAG_CAP) top = bot + BAG_CAP;
-    n = top - bot;
+    // ---- clamp the rings after overflow pushes, then spill the excess
+    my_top = lane < NCLS ? top_c[lane] : 0;
+    my_bot = lane < NCLS ? bot_c[lane] : 0;
+    if (lane < NCLS && my_top - my_bot > CLS_CAP) {
+      my_top = my_bot + CLS_CAP;
+      top_c[lane] = my_top;
+    }
+    my_n = my_top - my_bot;
+    n = my_n;
+#pragma unroll
+    for (u32 o = 1; o < NCLS; o <<= 1) n += __shfl_xor(n, o, 64);
+    n = uniform(n);
+    key = lane < NCLS ? (my_n << 2) | (NCLS - 1 - lane) : 0;
+#pragma unroll
+    for (u32 o = 1; o < NCLS; o <<= 1) {
+      u32 other = __shfl_xor(key, o, 64);
+      key = other > key ? other : key;
+    }
+    key = uniform(key);
+    const u32 sc = NCLS - 1 - (key & 3);
+    const u32 snc = key >> 2;
     u32 active_waves = (u32)(S_cache & 0xFFFFFFFFu);
     u32 idle_waves = P.total_waves > active_waves ? P.total_waves - active_waves : 0;
     u32 queued = (u32)(S_cache >> 32);
     u32 spill = 0;
-    if (n > BAG_CAP * 3 / 4) spill = n - BAG_CAP / 2;  // keep room for one iteration's pushes
-    else if (n > 64 + 16 && idle_waves > 0 && queued < idle_waves * 64) spill = n - 64;
+    if (snc > CLS_CAP * 3 / 4) spill = snc - CLS_CAP / 2;  // keep room for one iteration's pushes
+    else if (n > 64 + 16 && idle_waves > 0 && queued < idle_waves * 64) spill = snc < n - 64 ? snc : n - 64;
     if (spill) {
+      const u64 t_sp = wall_clock64();
       st[ST_SPILLS]++;
       st[ST_SPILLED] += spill;
-      u64 base = 0;
-      if (lane == 0) base = __hip_atomic_fetch_add(&g->q_tail, (u64)spill, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      base = __shfl(base, 0, 64);
-      for (u32 i = lane; i < spill; i += 64) {
-        u32 s2 = ((bot + i) & (BAG_CAP - 1)) * 2;
-        u64 *q = P.queue + ((base + i) & P.qmask) * 2;
-        gst(q, bag[s2]);
-        gst(q + 1, bag[s2 + 1]);
-      }
-      publish();
-      u64 old = 0;
-      if (lane == 0) {
-        old = __hip_atomic_fetch_add(&g->S, (u64)spill << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((old >> 32) + spill > (P.qmask + 1) / 2) fail(L, ERR_QUEUE);
-      }
-      S_cache = __shfl(old, 0, 64) + ((u64)spill << 32);
-      bot += spill;
-    }
-    if (lane == 0) {
-      s_top[wib] = top;
-      s_bot[wib] = bot;
+      u32 qerr = 0;
+      S_cache = spill_to_global(P, bag, &bot_c[sc], sc, spill, lane, uniform(__shfl(my_bot, sc, 64)), qerr);
+      if (qerr) fail(L, qerr);
+      st[ST_SPILL_TICKS] += wall_clock64() - t_sp;
     }
     __builtin_amdgcn_wave_barrier();
     st[ST_BUSY_TICKS] += wall_clock64() - t_it;
@@ -333,7 +385,6 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   wave_flush_counters(L, lane);
   __builtin_amdgcn_wave_barrier();
   if (lane == 0) {
-    P.alloc_pos[gwave] = s_alloc[wib];
     for (u32 k = 0; k < ST_SLOTS; k++)
       if (st[k]) __hip_atomic_fetch_add(&g->st[k], st[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }

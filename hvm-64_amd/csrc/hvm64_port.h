@@ -16,8 +16,11 @@
 #if defined(__HIPCC__) || defined(__HIP__)
 #include <hip/hip_runtime.h>
 #define HVM_HD __host__ __device__ __forceinline__
+// Rarely executed, large: kept out of line on the device (code size).
+#define HVM_HD_COLD __host__ __device__ inline __attribute__((noinline))
 #else
 #define HVM_HD inline
+#define HVM_HD_COLD inline
 #endif
 
 namespace hvm {
@@ -97,7 +100,7 @@ HVM_HD i32 sat_f2i(float f) {
   return (i32)f;
 }
 
-HVM_HD u32 operate_unary(u32 op, u32 num) {
+HVM_HD_COLD u32 operate_unary(u32 op, u32 num) {
   const u32 U24_MAX = (1u << 24) - 1;
   const i32 I24_MAX = (1 << 23) - 1, I24_MIN = -(1 << 23);
   u32 t = num_tag(num);
@@ -133,7 +136,7 @@ HVM_HD u32 operate_unary(u32 op, u32 num) {
 // `x` holds the partial operator (its payload is the left operand), `y` the
 // typed value; `*div0` is set on an integer division or remainder by zero,
 // which aborts the reference (Rust panic).
-HVM_HD u32 operate_typed(u32 op, u32 ty, u32 x, u32 y, bool *div0) {
+HVM_HD_COLD u32 operate_typed(u32 op, u32 ty, u32 x, u32 y, bool *div0) {
   if (ty == N_U24) {
     u32 p = num_u24(x), q = num_u24(y);
     switch (op) {
@@ -221,7 +224,7 @@ HVM_HD u32 operate_sym(u32 a, u32 b, bool *div0) {
   return aop ? operate_typed(at, bt, a, b, div0) : operate_typed(bt, at, b, a, div0);
 }
 
-HVM_HD u32 operate_binary(u32 a, u32 op, u32 b, bool *div0) {
+HVM_HD_COLD u32 operate_binary(u32 a, u32 op, u32 b, bool *div0) {
   if (op == N_SYM) return operate_sym(a, b, div0);
   return operate_sym(operate_sym(a, num_new_sym(op), div0), b, div0);
 }

@@ -33,19 +33,27 @@ namespace hvm {
 
 
 static const u32 WAVES_PER_BLOCK = 4;
-static const u32 BAG_CAP = 256;    // redexes per wave in LDS (power of two)
+// Every wave keeps its pending redexes in LDS, in one ring per rule class, so
+// that an iteration can run one rule on all 64 lanes.
+enum RedexClass : u32 { CLS_ANNI = 0, CLS_COMM = 1, CLS_CALL = 2, CLS_OTHER = 3, NCLS = 4 };
+static const u32 CLS_CAP = 128;    // redexes per class ring (power of two)
 static const u32 LDS_SLOTS = 10;   // trg registers per lane kept in LDS
+static const u32 LINK_CAP = 6;     // deferred links per lane (LDS)
 
 // ---------------------------------------------------------------------------
 // heap access
 // ---------------------------------------------------------------------------
 // Every heap address is checked: a bad port (an engine bug) is redirected to
 // the sink words and recorded, so it can never touch memory outside the heap.
+NOINL void report_error(Globals *g, u32 code) {
+  u32 zero = 0;
+  HVM_CAS(&g->err, &zero, code);
+}
+
 DEV u64 *hw(const Params &P, u64 addr) {
   u64 off = addr - HVM_HEAP_OFF;
   if (off >= P.heap_words * 8) {
-    u32 zero = 0;
-    HVM_CAS(&P.g->err, &zero, (u32)ERR_BADPTR);
+    report_error(P.g, ERR_BADPTR);
     off = DUMP_WORD * 8;
   }
   return P.heap + (off >> 3);
@@ -69,63 +77,123 @@ DEV u64 norm_endpoint(u64 p) { return is_endpoint(p) ? (paddr(p) | T_VAR) : p; }
 // ---------------------------------------------------------------------------
 struct Lane {
   const Params *P;
-  // allocator: the wave's ring of nodes [wave_node0, wave_node0 + P->wave_nodes)
-  u64 wave_node0;
-  u64 *alloc_ctr;      // the ring's position counter (LDS in the reduce kernel)
+  // allocator: this lane's slice of its wave's ring: nodes
+  // sub_node0 + 64*i for i in [0, P->wave_nodes / 64)
+  u64 sub_node0;
+  u64 apos;            // allocations from the slice since the heap was reset
+  u64 *apos_home;      // where apos is kept between launches
+  u32 slice_rest;      // allocations left to take from the overflow ring
+  u32 ov_left;         // nodes left in the current overflow run
+  u64 ov_cur;          // next overflow-ring position of that run
   u64 spare;           // reserved second word of a var pair, or 0
   bool dirty;          // stores of new nodes/cells not yet retired
+  // links deferred by the running definition: pair i at PL[2i * 64],
+  // PL[(2i + 1) * 64] (LDS; nullptr: link at once)
+  u64 *PL;
   // trg registers: T[s * 64] (LDS for small definitions, else global)
   u64 *T_lds;
   u64 *T_glb;
-  // LDS redex bag (nullptr: push straight to the global ring)
+  // LDS redex rings, NCLS x CLS_CAP pairs (nullptr: push straight to the
+  // global ring); per-class push counters and oldest entries
   u64 *bag;
   u32 *bag_top;
-  u32 bag_bot;
+  const u32 *bag_bot;
   // counters
   u32 rw[RW_SLOTS];
 };
 
-DEV void fail(Lane &L, u32 code) {
-  u32 zero = 0;
-  HVM_CAS(&L.P->g->err, &zero, code);
-}
+DEV void fail(Lane &L, u32 code) { report_error(L.P->g, code); }
 
 DEV u64 dump_addr() { return HVM_HEAP_OFF + DUMP_WORD * 8; }
 
-// Ring allocation.  Every wave owns a ring of 2^k nodes; the ring position
-// counts allocations since the heap was reset, so on the first lap every slot
-// is known to be free and is taken without reading it.  Later laps take the
-// next slot whose two words are FREE (frees write FREE in place, from any
-// wave).  A ring with no free slot in a full lap falls back to the shared
-// overflow ring, which works the same way.
-DEV u64 ring_take(Lane &L, u64 *ctr, u64 node0, u64 nodes, bool &ok) {
-  const Params &P = *L.P;
-  for (u64 miss = 0; miss <= nodes; miss++) {
-    u64 pos = HVM_FAA(ctr, 1ull);
-    u64 addr = HVM_HEAP_OFF + (node0 + (pos & (nodes - 1))) * 16;
-    if (pos < nodes) {
-      ok = true;
-      return addr;
+// Ring allocation.  Every wave owns a ring of 2^k nodes and every lane a
+// 1/64 slice of it (every 64th node, so lanes allocating together write
+// adjacent nodes).  A slice's position counts its allocations since the heap
+// was reset, so on the first lap every slot is known to be free and is taken
+// without reading it.  Later laps take the next slot whose two words are
+// FREE (frees write FREE in place, from any wave).  When SCAN_MISSES slots in
+// a row are taken, the slice is treated as full for the next SLICE_REST
+// allocations, which come from the shared overflow ring in runs of OV_RUN
+// nodes (one atomic per run; a run in the overflow ring's first lap is free
+// without reads, later laps check every slot).
+static const u32 SCAN_MISSES = 8;
+static const u32 SLICE_REST = 256;
+static const u32 OV_RUN = 64;
+
+// The slow path takes and returns the cursor by value (the lane state stays
+// in registers).
+struct AllocCursor {
+  u64 apos, ov_cur;
+  u32 slice_rest, ov_left;
+  u32 misses, ov_allocs;  // counter increments
+  u64 addr;               // the node allocated (dump_addr() and err on OOM)
+  u32 err;
+};
+
+// Ring slots are in the heap by construction of the layout (no bounds check).
+NOINL AllocCursor alloc_slow(u64 *heap, Globals *g, u64 wave_nodes, u64 ov_node0, u64 ov_nodes, u64 sub_node0,
+                             AllocCursor c) {
+  const u64 n = wave_nodes / 64;
+  c.err = 0;
+  if (c.slice_rest == 0) {
+    for (u32 miss = 0; miss < SCAN_MISSES; miss++) {
+      u64 node = sub_node0 + (c.apos & (n - 1)) * 64;
+      c.apos++;
+      u64 w0 = HVM_LD(heap + 2 * node), w1 = HVM_LD(heap + 2 * node + 1);
+      if (w0 == P_FREE && w1 == P_FREE) {
+        c.addr = HVM_HEAP_OFF + node * 16;
+        return c;
+      }
+      c.misses++;
     }
-    u64 w0 = ld(P, addr), w1 = ld(P, addr + 8);
+    c.slice_rest = SLICE_REST;
+  } else {
+    c.slice_rest--;
+  }
+  c.ov_allocs++;
+  for (u64 miss = 0; miss <= ov_nodes; miss++) {
+    if (c.ov_left == 0) {
+      c.ov_cur = HVM_FAA(&g->ov_pos, (u64)OV_RUN);
+      c.ov_left = OV_RUN;
+    }
+    u64 pos = c.ov_cur++;
+    c.ov_left--;
+    u64 node = ov_node0 + (pos & (ov_nodes - 1));
+    if (pos < ov_nodes) {
+      c.addr = HVM_HEAP_OFF + node * 16;
+      return c;
+    }
+    u64 w0 = HVM_LD(heap + 2 * node), w1 = HVM_LD(heap + 2 * node + 1);
     if (w0 == P_FREE && w1 == P_FREE) {
-      ok = true;
-      return addr;
+      c.addr = HVM_HEAP_OFF + node * 16;
+      return c;
     }
   }
-  ok = false;
-  return dump_addr();
+  c.err = ERR_OOM;
+  c.addr = dump_addr();
+  return c;
 }
 
-NOINL u64 alloc_node(Lane &L) {
+DEV u64 alloc_node(Lane &L) {
+  const u64 n = L.P->wave_nodes / 64;
+  if (L.apos < n) return HVM_HEAP_OFF + (L.sub_node0 + (L.apos++) * 64) * 16;
+  AllocCursor c;
+  c.apos = L.apos;
+  c.ov_cur = L.ov_cur;
+  c.slice_rest = L.slice_rest;
+  c.ov_left = L.ov_left;
+  c.misses = 0;
+  c.ov_allocs = 0;
   const Params &P = *L.P;
-  bool ok;
-  u64 a = ring_take(L, L.alloc_ctr, L.wave_node0, P.wave_nodes, ok);
-  if (ok) return a;
-  a = ring_take(L, &P.g->ov_pos, P.ov_node0, P.ov_nodes, ok);
-  if (ok) return a;
-  fail(L, ERR_OOM);
-  return dump_addr();
+  c = alloc_slow(P.heap, P.g, P.wave_nodes, P.ov_node0, P.ov_nodes, L.sub_node0, c);
+  L.apos = c.apos;
+  L.ov_cur = c.ov_cur;
+  L.slice_rest = c.slice_rest;
+  L.ov_left = c.ov_left;
+  L.rw[RW_ALLOC_MISS] += c.misses;
+  L.rw[RW_OV_ALLOC] += c.ov_allocs;
+  if (c.err) fail(L, c.err);
+  return c.addr;
 }
 
 // Var cells come in pairs from one node; both words are reserved at once.
@@ -171,16 +239,36 @@ DEV void publish_if_dirty(Lane &L) {
 // ---------------------------------------------------------------------------
 // redexes
 // ---------------------------------------------------------------------------
-NOINL void global_push1(Lane &L, u64 a, u64 b) {
-  const Params &P = *L.P;
-  publish_if_dirty(L);
-  u64 pos = HVM_FAA(&P.g->q_tail, 1ull);
-  u64 *s = P.queue + (pos & P.qmask) * 2;
+// Push one redex to the global ring (retires the lane's stores first).
+NOINL void push_global(u64 *queue, u64 qmask, Globals *g, u64 a, u64 b) {
+  publish();
+  u64 pos = HVM_FAA(&g->q_tail, 1ull);
+  u64 *s = queue + (pos & qmask) * 2;
   gst(s, a);
   gst(s + 1, b);
   publish();
-  u64 old = HVM_FAA(&P.g->S, 1ull << 32);
-  if ((old >> 32) + 1 > (P.qmask + 1) / 2) fail(L, ERR_QUEUE);
+  u64 old = HVM_FAA(&g->S, 1ull << 32);
+  if ((old >> 32) + 1 > (qmask + 1) / 2) report_error(g, ERR_QUEUE);
+}
+
+DEV void global_push1(Lane &L, u64 a, u64 b) {
+  push_global(L.P->queue, L.P->qmask, L.P->g, a, b);
+  L.dirty = false;
+}
+
+// The rule a redex will take, grouped: the arms of interact() that run
+// anni2, comm22, a definition call (a non-ERA Ref), and everything else.
+DEV u32 redex_class(u64 a, u64 b) {
+  u32 at = ptag(a), bt = ptag(b);
+  if ((at == T_REF && a != P_ERA) || (bt == T_REF && b != P_ERA)) return CLS_CALL;
+  if (at == bt) {
+    if (at == T_OP || at == T_SWI) return CLS_ANNI;
+    if (at == T_CTR) return plab(a) == plab(b) ? CLS_ANNI : CLS_COMM;
+    return CLS_OTHER;
+  }
+  if ((at == T_CTR && bt == T_OP) || (at == T_OP && bt == T_CTR)) return CLS_COMM;
+  if ((at == T_CTR && bt == T_SWI && plab(a) != 0) || (at == T_SWI && bt == T_CTR && plab(b) != 0)) return CLS_COMM;
+  return CLS_OTHER;
 }
 
 // linker.rs:194-209: skippable pairs count as an erasure and are dropped.
@@ -190,9 +278,10 @@ DEV void redux(Lane &L, u64 a, u64 b) {
     return;
   }
   if (L.bag) {
-    u32 idx = HVM_LDS_FAA(L.bag_top, 1u);
-    if (idx - L.bag_bot < BAG_CAP) {
-      u32 k = (idx & (BAG_CAP - 1)) * 2;
+    u32 c = redex_class(a, b);
+    u32 idx = HVM_LDS_FAA(&L.bag_top[c], 1u);
+    if (idx - L.bag_bot[c] < CLS_CAP) {
+      u32 k = (c * CLS_CAP + (idx & (CLS_CAP - 1))) * 2;
       L.bag[k] = a;
       L.bag[k + 1] = b;
       return;
@@ -223,21 +312,33 @@ DEV bool link_begin(Lane &L, u64 &a, u64 &b, u64 &cell) {
   return true;
 }
 
-NOINL void link(Lane &L, u64 a, u64 b) {
+// Continues a link after its swap found the other endpoint's port `old` in
+// `cell`: the cell is ours to free, and `old` is linked on to `b`.
+DEV void link_more(Lane &L, u64 cell, u64 old, u64 b) {
   const Params &P = *L.P;
+  st(P, cell, P_FREE);
+  u64 a = old;
   for (u32 hop = 0;; hop++) {
     if (hop > MAX_CHAIN) {
       fail(L, ERR_BADPTR);
       return;
     }
-    u64 cell;
-    if (!link_begin(L, a, b, cell)) return;
+    u64 c;
+    if (!link_begin(L, a, b, c)) return;
     publish_if_dirty(L);
-    u64 old = xchg(P, cell, norm_endpoint(b));
-    if (old == P_NONE) return;
-    st(P, cell, P_FREE);
-    a = old;
+    u64 o = xchg(P, c, norm_endpoint(b));
+    if (o == P_NONE) return;
+    st(P, c, P_FREE);
+    a = o;
   }
+}
+
+DEV void link(Lane &L, u64 a, u64 b) {
+  u64 cell;
+  if (!link_begin(L, a, b, cell)) return;
+  publish_if_dirty(L);
+  u64 old = xchg(*L.P, cell, norm_endpoint(b));
+  if (old != P_NONE) link_more(L, cell, old, b);
 }
 
 // Two / four independent links whose first swaps are in flight together.
@@ -249,14 +350,8 @@ DEV void link2(Lane &L, u64 a1, u64 b1, u64 a2, u64 b2) {
   if (p1 || p2) publish_if_dirty(L);
   u64 o1 = p1 ? xchg(P, c1, norm_endpoint(b1)) : P_NONE;
   u64 o2 = p2 ? xchg(P, c2, norm_endpoint(b2)) : P_NONE;
-  if (o1 != P_NONE) {
-    st(P, c1, P_FREE);
-    link(L, o1, b1);
-  }
-  if (o2 != P_NONE) {
-    st(P, c2, P_FREE);
-    link(L, o2, b2);
-  }
+  if (o1 != P_NONE) link_more(L, c1, o1, b1);
+  if (o2 != P_NONE) link_more(L, c2, o2, b2);
 }
 
 DEV void link4(Lane &L, u64 a1, u64 b1, u64 a2, u64 b2, u64 a3, u64 b3, u64 a4, u64 b4) {
@@ -271,16 +366,35 @@ DEV void link4(Lane &L, u64 a1, u64 b1, u64 a2, u64 b2, u64 a3, u64 b3, u64 a4, 
   u64 o2 = p2 ? xchg(P, c2, norm_endpoint(b2)) : P_NONE;
   u64 o3 = p3 ? xchg(P, c3, norm_endpoint(b3)) : P_NONE;
   u64 o4 = p4 ? xchg(P, c4, norm_endpoint(b4)) : P_NONE;
-  if (o1 != P_NONE) { st(P, c1, P_FREE); link(L, o1, b1); }
-  if (o2 != P_NONE) { st(P, c2, P_FREE); link(L, o2, b2); }
-  if (o3 != P_NONE) { st(P, c3, P_FREE); link(L, o3, b3); }
-  if (o4 != P_NONE) { st(P, c4, P_FREE); link(L, o4, b4); }
+  if (o1 != P_NONE) link_more(L, c1, o1, b1);
+  if (o2 != P_NONE) link_more(L, c2, o2, b2);
+  if (o3 != P_NONE) link_more(L, c3, o3, b3);
+  if (o4 != P_NONE) link_more(L, c4, o4, b4);
+}
+
+// The first `cnt` of four links, their swaps in flight together.
+DEV void link_n(Lane &L, u32 cnt, u64 a1, u64 b1, u64 a2, u64 b2, u64 a3, u64 b3, u64 a4, u64 b4) {
+  const Params &P = *L.P;
+  u64 c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+  bool p1 = link_begin(L, a1, b1, c1);
+  bool p2 = cnt > 1 && link_begin(L, a2, b2, c2);
+  bool p3 = cnt > 2 && link_begin(L, a3, b3, c3);
+  bool p4 = cnt > 3 && link_begin(L, a4, b4, c4);
+  if (p1 || p2 || p3 || p4) publish_if_dirty(L);
+  u64 o1 = p1 ? xchg(P, c1, norm_endpoint(b1)) : P_NONE;
+  u64 o2 = p2 ? xchg(P, c2, norm_endpoint(b2)) : P_NONE;
+  u64 o3 = p3 ? xchg(P, c3, norm_endpoint(b3)) : P_NONE;
+  u64 o4 = p4 ? xchg(P, c4, norm_endpoint(b4)) : P_NONE;
+  if (o1 != P_NONE) link_more(L, c1, o1, b1);
+  if (o2 != P_NONE) link_more(L, c2, o2, b2);
+  if (o3 != P_NONE) link_more(L, c3, o3, b3);
+  if (o4 != P_NONE) link_more(L, c4, o4, b4);
 }
 
 // Follow a wire endpoint whose other side has already linked (the
 // reference's Trg::target on a wire, linker.rs:401-404).  The caller owns the
 // endpoint, so once a cell holds a port the cell is ours to free.
-NOINL u64 resolve(Lane &L, u64 p) {
+DEV u64 resolve(Lane &L, u64 p) {
   const Params &P = *L.P;
   for (u32 hop = 0; is_endpoint(p); hop++) {
     if (hop > MAX_CHAIN) {
@@ -301,7 +415,7 @@ DEV u64 as_wire(u64 p) { return is_endpoint(p) ? wire_port(paddr(p)) : p; }
 // ---------------------------------------------------------------------------
 // interaction rules (interact.rs)
 // ---------------------------------------------------------------------------
-NOINL void anni2(Lane &L, u64 a, u64 b) {
+DEV void anni2(Lane &L, u64 a, u64 b) {
   L.rw[RW_ANNI]++;
   u64 a1, a2, b1, b2;
   take(L, a, a1, a2);
@@ -309,7 +423,7 @@ NOINL void anni2(Lane &L, u64 a, u64 b) {
   link2(L, a1, b1, a2, b2);
 }
 
-NOINL void comm22(Lane &L, u64 a, u64 b) {
+DEV void comm22(Lane &L, u64 a, u64 b) {
   L.rw[RW_COMM]++;
   u64 a1, a2, b1, b2;
   take(L, a, a1, a2);
@@ -322,14 +436,14 @@ NOINL void comm22(Lane &L, u64 a, u64 b) {
   link4(L, a1, B1, a2, B2, b1, A1, b2, A2);
 }
 
-NOINL void comm02(Lane &L, u64 a, u64 b) {
+DEV void comm02(Lane &L, u64 a, u64 b) {
   L.rw[RW_COMM]++;
   u64 b1, b2;
   take(L, b, b1, b2);
   link2(L, b1, a, b2, a);
 }
 
-NOINL void switch_num(Lane &L, u64 a, u64 b) {  // interact.rs:201-223
+DEV void switch_num(Lane &L, u64 a, u64 b) {  // interact.rs:201-223
   L.rw[RW_OPER]++;
   u64 a1, a2;
   take(L, a, a1, a2);
@@ -345,7 +459,7 @@ NOINL void switch_num(Lane &L, u64 a, u64 b) {  // interact.rs:201-223
   }
 }
 
-NOINL void op_num(Lane &L, u64 a, u64 b) {  // interact.rs:251-270
+DEV void op_num(Lane &L, u64 a, u64 b) {  // interact.rs:251-270
   u32 op = plab(a);
   u64 a1, a2;
   take(L, a, a1, a2);
@@ -373,110 +487,173 @@ DEV bool labs_has(const Params &P, const DefRec &d, u32 lab) {  // def.rs:30-37
 }
 
 
-// instruction.rs:403-422
-NOINL void do_ctr(Lane &L, u32 lab, u64 trg, u64 &o1, u64 &o2) {
-  u64 port = ptag(trg) == T_RED ? resolve(L, trg) : trg;
-  if (ptag(port) == T_CTR && plab(port) == lab) {
-    u64 x, y;
-    take(L, port, x, y);
-    L.rw[RW_ANNI]++;
-    o1 = as_wire(x);
-    o2 = as_wire(y);
-    return;
-  }
-  u64 v1 = alloc_var(L), v2 = alloc_var(L);
-  u64 n = new_node(L, T_CTR, lab, var_port(v1), var_port(v2));
-  link(L, port, n);
-  o1 = var_port(v1);
-  o2 = var_port(v2);
-}
-
-// instruction.rs:425-442
-NOINL void do_op(Lane &L, u32 op, u64 trg, u64 &o1, u64 &o2) {
-  u64 port = ptag(trg) == T_RED ? resolve(L, trg) : trg;
-  if (ptag(port) == T_NUM) {
-    u64 v = alloc_var(L);
-    u64 n = new_node(L, T_OP, N_SYM, numport(operate_unary(op, pnum(port))), var_port(v));
-    o1 = n;
-    o2 = var_port(v);
-    return;
-  }
-  if (port == P_ERA) {
-    L.rw[SC_OP_ERA]++;
-    o1 = P_ERA;
-    o2 = P_ERA;
-    return;
-  }
-  u64 v1 = alloc_var(L), v2 = alloc_var(L);
-  u64 n = new_node(L, T_OP, op, var_port(v1), var_port(v2));
-  link(L, port, n);
-  o1 = var_port(v1);
-  o2 = var_port(v2);
-}
-
-// instruction.rs:445-462
-NOINL u64 do_op_num(Lane &L, u32 op, u64 trg, u32 rhs) {
-  u64 port = ptag(trg) == T_RED ? resolve(L, trg) : trg;
-  if (ptag(port) == T_NUM) {
-    L.rw[RW_OPER]++;
-    bool div0 = false;
-    u32 out = operate_binary(pnum(port), op, rhs, &div0);
-    if (div0) fail(L, ERR_DIV0);
-    return numport(out);
-  }
-  if (port == P_ERA) {
-    L.rw[SC_OPNUM_ERA]++;
-    return P_ERA;
-  }
-  u64 v = alloc_var(L);
-  u64 n = new_node(L, T_OP, op, numport(rhs), var_port(v));
-  link(L, port, n);
-  return var_port(v);
-}
-
-// instruction.rs:465-493
-NOINL void do_switch(Lane &L, u64 trg, u64 &o1, u64 &o2) {
-  u64 port = ptag(trg) == T_RED ? resolve(L, trg) : trg;
-  if (ptag(port) == T_NUM) {
-    L.rw[RW_OPER]++;
-    u32 n = pnum(port);
-    u32 v = num_tag(n) == N_U24 ? num_u24(n) : 0;
-    u64 w = alloc_var(L);
-    if (v == 0) {
-      o1 = new_node(L, T_CTR, 0, var_port(w), P_ERA);
-    } else {
-      u64 c2 = new_node(L, T_CTR, 0, numport(num_new_u24(v - 1)), var_port(w));
-      o1 = new_node(L, T_CTR, 0, P_ERA, c2);
+// Definition instructions (def.rs:217-289, instruction.rs:390-493).
+//
+// A definition runs as a straight sequence of steps with the same shape for
+// every instruction kind, so that lanes running different definitions stay
+// converged on the memory operations: (1) the trg operand is read and, if
+// it is a wire, followed (Trg::target); (2) a CTR meeting a CTR of its own
+// label takes that node (the annihilation fast path of do_ctr); (3) the
+// kind's own work, which only allocates and stores; (4) the links it asks
+// for are appended to a per-lane list.  The list is linked four at a time
+// when it fills and when the definition ends.  Deferring a link inside a
+// definition changes nothing a later instruction can rely on: a trg read in
+// (1) sees either the port a link would have parked or the unresolved wire,
+// and both lead to the same net with the same rewrite counts (the fast
+// paths count what the interaction they replace counts).
+DEV void flush_links(Lane &L, u32 &np) {
+  u64 *PL = L.PL;
+  for (u32 i = 0; i < np; i += 4) {
+    u32 cnt = np - i < 4 ? np - i : 4;
+    u64 a[4], b[4];
+#pragma unroll
+    for (u32 j = 0; j < 4; j++) {
+      a[j] = j < cnt ? PL[(u64)(2 * (i + j)) * 64] : 0;
+      b[j] = j < cnt ? PL[(u64)(2 * (i + j) + 1) * 64] : 0;
     }
-    o2 = wire_port(w);
-    return;
+    link_n(L, cnt, a[0], b[0], a[1], b[1], a[2], b[2], a[3], b[3]);
   }
-  if (port == P_ERA) {
-    L.rw[SC_SWITCH_ERA]++;
-    L.rw[RW_ERAS]++;
-    o1 = P_ERA;
-    o2 = P_ERA;
-    return;
-  }
-  u64 v1 = alloc_var(L), v2 = alloc_var(L);
-  u64 m = new_node(L, T_SWI, 0, var_port(v1), var_port(v2));
-  link(L, port, m);
-  o1 = var_port(v1);
-  o2 = var_port(v2);
+  np = 0;
 }
+
+DEV void push_link(Lane &L, u32 &np, u64 a, u64 b) {
+  if (!L.PL) {
+    link(L, a, b);
+    return;
+  }
+  if (np == LINK_CAP) {  // list full: this one links at once
+    link(L, a, b);
+    return;
+  }
+  L.PL[(u64)(2 * np) * 64] = a;
+  L.PL[(u64)(2 * np + 1) * 64] = b;
+  np++;
+}
+
+#define T_AT(s) T[(u64)(s) * 64]
+DEV void exec_step(Lane &L, u64 *T, u32 lim, u64 w0, u64 w1, u32 &np) {
+  const u32 kind = (u32)(w0 & 0xFF);
+  const u32 a = (u32)(w0 >> 16) & 0xFFFF, b = (u32)(w0 >> 32) & 0xFFFF, c = (u32)(w0 >> 48);
+  const u32 d4 = kind == I_WIRES ? (u32)(w1 & 0xFFFF) : 0;
+  if (a >= lim || b >= lim || c >= lim || d4 >= lim || kind > I_WIRES) {
+    fail(L, kind > I_WIRES ? ERR_UNIMPL : ERR_BADPTR);
+    return;
+  }
+  const u32 lab = (u32)(w1 & 0xFFFF);
+  // (1) operands
+  const u64 ta = (kind == I_CONST || kind == I_WIRES) ? 0 : T_AT(a);
+  const bool wants_port = kind >= I_CTR && kind <= I_SWITCH;
+  u64 port = ta;
+  if (wants_port && ptag(ta) == T_RED) port = resolve(L, ta);
+  // (2) do_ctr's annihilation fast path (instruction.rs:406-410)
+  const bool fast_ctr = kind == I_CTR && ptag(port) == T_CTR && plab(port) == lab;
+  u64 x = 0, y = 0;
+  if (fast_ctr) take(L, port, x, y);
+  // (3) + (4)
+  switch (kind) {
+    case I_CONST: T_AT(a) = w1; break;
+    case I_LINK: push_link(L, np, ta, T_AT(b)); break;
+    case I_LINKCONST: push_link(L, np, ta, w1); break;
+    case I_CTR: {  // instruction.rs:403-422
+      if (fast_ctr) {
+        L.rw[RW_ANNI]++;
+        T_AT(b) = as_wire(x);
+        T_AT(c) = as_wire(y);
+        break;
+      }
+      u64 v1 = alloc_var(L), v2 = alloc_var(L);
+      push_link(L, np, port, new_node(L, T_CTR, lab, var_port(v1), var_port(v2)));
+      T_AT(b) = var_port(v1);
+      T_AT(c) = var_port(v2);
+      break;
+    }
+    case I_OP: {  // instruction.rs:425-442
+      if (ptag(port) == T_NUM) {
+        u64 v = alloc_var(L);
+        T_AT(b) = new_node(L, T_OP, N_SYM, numport(operate_unary(lab, pnum(port))), var_port(v));
+        T_AT(c) = var_port(v);
+      } else if (port == P_ERA) {
+        L.rw[SC_OP_ERA]++;
+        T_AT(b) = P_ERA;
+        T_AT(c) = P_ERA;
+      } else {
+        u64 v1 = alloc_var(L), v2 = alloc_var(L);
+        push_link(L, np, port, new_node(L, T_OP, lab, var_port(v1), var_port(v2)));
+        T_AT(b) = var_port(v1);
+        T_AT(c) = var_port(v2);
+      }
+      break;
+    }
+    case I_OPNUM: {  // instruction.rs:445-462
+      const u32 rhs = (u32)(w1 >> 32);
+      if (ptag(port) == T_NUM) {
+        L.rw[RW_OPER]++;
+        bool div0 = false;
+        u32 out = operate_binary(pnum(port), lab, rhs, &div0);
+        if (div0) fail(L, ERR_DIV0);
+        T_AT(c) = numport(out);
+      } else if (port == P_ERA) {
+        L.rw[SC_OPNUM_ERA]++;
+        T_AT(c) = P_ERA;
+      } else {
+        u64 v = alloc_var(L);
+        push_link(L, np, port, new_node(L, T_OP, lab, numport(rhs), var_port(v)));
+        T_AT(c) = var_port(v);
+      }
+      break;
+    }
+    case I_SWITCH: {  // instruction.rs:465-493
+      if (ptag(port) == T_NUM) {
+        L.rw[RW_OPER]++;
+        u32 n = pnum(port);
+        u32 v = num_tag(n) == N_U24 ? num_u24(n) : 0;
+        u64 w = alloc_var(L);
+        if (v == 0) {
+          T_AT(b) = new_node(L, T_CTR, 0, var_port(w), P_ERA);
+        } else {
+          u64 c2 = new_node(L, T_CTR, 0, numport(num_new_u24(v - 1)), var_port(w));
+          T_AT(b) = new_node(L, T_CTR, 0, P_ERA, c2);
+        }
+        T_AT(c) = wire_port(w);
+      } else if (port == P_ERA) {
+        L.rw[SC_SWITCH_ERA]++;
+        L.rw[RW_ERAS]++;
+        T_AT(b) = P_ERA;
+        T_AT(c) = P_ERA;
+      } else {
+        u64 v1 = alloc_var(L), v2 = alloc_var(L);
+        push_link(L, np, port, new_node(L, T_SWI, 0, var_port(v1), var_port(v2)));
+        T_AT(b) = var_port(v1);
+        T_AT(c) = var_port(v2);
+      }
+      break;
+    }
+    case I_WIRES: {
+      u64 v1 = alloc_var(L), v2 = alloc_var(L);
+      T_AT(a) = var_port(v1);
+      T_AT(b) = wire_port(v1);
+      T_AT(c) = var_port(v2);
+      T_AT(d4) = wire_port(v2);
+      break;
+    }
+  }
+}
+#undef T_AT
 
 // def.rs:217-289.  Instructions are fetched four at a time (one 64-byte
 // line); small definitions keep their trg registers in LDS.
-NOINL void exec_instr(Lane &L, u64 *T, u64 w0, u64 w1);
-NOINL void interpret(Lane &L, u32 di, u64 t0) {
+DEV void interpret(Lane &L, u32 di, u64 t0) {
   const Params &P = *L.P;
   const DefRec d = P.defs[di];
-  u64 *T = (d.nslots <= LDS_SLOTS && L.T_lds) ? L.T_lds : L.T_glb;
-  if (d.nslots > P.trg_slots && T == L.T_glb) {
+  const bool in_lds = d.nslots <= LDS_SLOTS && L.T_lds;
+  u64 *T = in_lds ? L.T_lds : L.T_glb;
+  const u32 lim = in_lds ? LDS_SLOTS : P.trg_slots;
+  if (d.nslots > lim) {
     fail(L, ERR_BADPTR);
     return;
   }
   T[0] = t0;
+  u32 np = 0;
   const u64 *ip = P.instrs + (u64)d.ioff * 2;
   for (u32 k = 0; k < d.ilen; k += 4) {
     u64 w[8];
@@ -487,66 +664,15 @@ NOINL void interpret(Lane &L, u32 di, u64 t0) {
       w[2 * j + 1] = in ? ip[2 * (k + j) + 1] : 0;
     }
     u32 cnt = d.ilen - k < 4 ? d.ilen - k : 4;
-    for (u32 j = 0; j < cnt; j++) exec_instr(L, T, w[2 * j], w[2 * j + 1]);
+    for (u32 j = 0; j < cnt; j++) exec_step(L, T, lim, w[2 * j], w[2 * j + 1], np);
   }
+  if (np) flush_links(L, np);
 }
-
-#define T_SET(s, v) (T[(u64)(s) * 64] = (v))
-NOINL void exec_instr(Lane &L, u64 *T, u64 w0, u64 w1) {
-  const Params &P = *L.P;
-  {
-    u32 kind = (u32)(w0 & 0xFF);
-    u32 a = (u32)(w0 >> 16) & 0xFFFF, b = (u32)(w0 >> 32) & 0xFFFF, c = (u32)(w0 >> 48);
-    u32 d4 = kind == I_WIRES ? (u32)(w1 & 0xFFFF) : 0;
-    u32 lim = T == L.T_lds ? LDS_SLOTS : P.trg_slots;
-    if (a >= lim || b >= lim || c >= lim || d4 >= lim) {
-      fail(L, ERR_BADPTR);
-      return;
-    }
-    switch (kind) {
-      case I_CONST: T_SET(a, w1); break;
-      case I_LINK: link(L, T[(u64)a * 64], T[(u64)b * 64]); break;
-      case I_LINKCONST: link(L, T[(u64)a * 64], w1); break;
-      case I_CTR: {
-        u64 x, y;
-        do_ctr(L, (u32)(w1 & 0xFFFF), T[(u64)a * 64], x, y);
-        T_SET(b, x);
-        T_SET(c, y);
-        break;
-      }
-      case I_OP: {
-        u64 x, y;
-        do_op(L, (u32)(w1 & 0xFFFF), T[(u64)a * 64], x, y);
-        T_SET(b, x);
-        T_SET(c, y);
-        break;
-      }
-      case I_OPNUM: T_SET(c, do_op_num(L, (u32)(w1 & 0xFFFF), T[(u64)a * 64], (u32)(w1 >> 32))); break;
-      case I_SWITCH: {
-        u64 x, y;
-        do_switch(L, T[(u64)a * 64], x, y);
-        T_SET(b, x);
-        T_SET(c, y);
-        break;
-      }
-      case I_WIRES: {
-        u64 v1 = alloc_var(L), v2 = alloc_var(L);
-        T_SET(a, var_port(v1));
-        T_SET(b, wire_port(v1));
-        T_SET(c, var_port(v2));
-        T_SET((u32)(w1 & 0xFFFF), wire_port(v2));
-        break;
-      }
-      default: fail(L, ERR_UNIMPL); return;
-    }
-  }
-}
-#undef T_SET
 
 DEV u64 expand_ref(u64 cell) { return mkport(T_REF, 0xFFFF, cell); }
 
 // net.rs:353-380.  `cell` holds the ExpandDef's `out` endpoint.
-NOINL void expand_call(Lane &L, u64 cell, u64 port) {
+DEV void expand_call(Lane &L, u64 cell, u64 port) {
   const Params &P = *L.P;
   if (ptag(port) == T_REF && port != P_ERA) {
     u64 oaddr = paddr(port);
@@ -585,7 +711,7 @@ NOINL void expand_call(Lane &L, u64 cell, u64 port) {
 }
 
 // def.rs:177-189
-NOINL void call(Lane &L, u64 port, u64 trg) {
+DEV void call(Lane &L, u64 port, u64 trg) {
   const Params &P = *L.P;
   u64 addr = paddr(port);
   if (addr_in_heap(addr)) {  // ExpandDef: LabSet::ALL
@@ -611,48 +737,78 @@ NOINL void call(Lane &L, u64 port, u64 trg) {
   interpret(L, di, trg);
 }
 
-// interact.rs:6-44 (arm order preserved)
-NOINL void interact(Lane &L, u64 a, u64 b) {
+// interact.rs:6-44.  The arms are tested in the reference's order; each arm
+// picks a rule and which side goes first, and every rule has a single call
+// site (code size: the rules are inlined once).
+enum Rule : u32 { R_UNIMPL, R_CALL, R_ERAS, R_COMM22, R_ANNI, R_COMM02, R_OPNUM, R_SWITCHNUM };
+
+DEV u32 choose_rule(u64 a, u64 b, bool &swap) {
   u32 at = ptag(a), bt = ptag(b);
   bool anil = at == T_REF || at == T_NUM, bnil = bt == T_REF || bt == T_NUM;
-  if (at <= T_VAR || bt <= T_VAR) { fail(L, ERR_UNIMPL); return; }
-  if (at == T_REF && bnil && !is_skippable(a)) { call(L, a, b); return; }
-  if (anil && bt == T_REF && !is_skippable(b)) { call(L, b, a); return; }
-  if (anil && bnil) { L.rw[RW_ERAS]++; return; }
-  if (at == T_CTR && bt == T_SWI && plab(a) != 0) { comm22(L, a, b); return; }
-  if (at == T_SWI && bt == T_CTR && plab(b) != 0) { comm22(L, a, b); return; }
-  if ((at == T_CTR && bt == T_OP) || (at == T_OP && bt == T_CTR)) { comm22(L, a, b); return; }
-  if (at == T_CTR && bt == T_CTR && plab(a) != plab(b)) { comm22(L, a, b); return; }
-  if (at == bt && (at == T_SWI || at == T_OP || at == T_CTR)) { anni2(L, a, b); return; }
-  if (at == T_REF && bt == T_CTR && plab(b) >= plab(a)) { comm02(L, a, b); return; }
-  if (at == T_CTR && bt == T_REF && plab(a) >= plab(b)) { comm02(L, b, a); return; }
-  if (at == T_NUM && bt == T_CTR) { comm02(L, a, b); return; }
-  if (at == T_CTR && bt == T_NUM) { comm02(L, b, a); return; }
-  if (a == P_ERA) { comm02(L, a, b); return; }
-  if (b == P_ERA) { comm02(L, b, a); return; }
-  if (at == T_REF) { call(L, a, b); return; }
-  if (bt == T_REF) { call(L, b, a); return; }
-  if (at == T_OP && bt == T_NUM) { op_num(L, a, b); return; }
-  if (at == T_NUM && bt == T_OP) { op_num(L, b, a); return; }
-  if (at == T_SWI && bt == T_NUM) { switch_num(L, a, b); return; }
-  if (at == T_NUM && bt == T_SWI) { switch_num(L, b, a); return; }
-  fail(L, ERR_UNIMPL);  // Switch-Ctr(0), Op-Switch: unimplemented!() in the reference
+  swap = false;
+  if (at <= T_VAR || bt <= T_VAR) return R_UNIMPL;
+  if (at == T_REF && bnil && !is_skippable(a)) return R_CALL;
+  if (anil && bt == T_REF && !is_skippable(b)) { swap = true; return R_CALL; }
+  if (anil && bnil) return R_ERAS;
+  if (at == T_CTR && bt == T_SWI && plab(a) != 0) return R_COMM22;
+  if (at == T_SWI && bt == T_CTR && plab(b) != 0) return R_COMM22;
+  if ((at == T_CTR && bt == T_OP) || (at == T_OP && bt == T_CTR)) return R_COMM22;
+  if (at == T_CTR && bt == T_CTR && plab(a) != plab(b)) return R_COMM22;
+  if (at == bt && (at == T_SWI || at == T_OP || at == T_CTR)) return R_ANNI;
+  if (at == T_REF && bt == T_CTR && plab(b) >= plab(a)) return R_COMM02;
+  if (at == T_CTR && bt == T_REF && plab(a) >= plab(b)) { swap = true; return R_COMM02; }
+  if (at == T_NUM && bt == T_CTR) return R_COMM02;
+  if (at == T_CTR && bt == T_NUM) { swap = true; return R_COMM02; }
+  if (a == P_ERA) return R_COMM02;
+  if (b == P_ERA) { swap = true; return R_COMM02; }
+  if (at == T_REF) return R_CALL;
+  if (bt == T_REF) { swap = true; return R_CALL; }
+  if (at == T_OP && bt == T_NUM) return R_OPNUM;
+  if (at == T_NUM && bt == T_OP) { swap = true; return R_OPNUM; }
+  if (at == T_SWI && bt == T_NUM) return R_SWITCHNUM;
+  if (at == T_NUM && bt == T_SWI) { swap = true; return R_SWITCHNUM; }
+  return R_UNIMPL;  // Switch-Ctr(0), Op-Switch: unimplemented!() in the reference
+}
+
+DEV void interact(Lane &L, u64 a, u64 b) {
+  bool swap;
+  const u32 rule = choose_rule(a, b, swap);
+  if (swap) {
+    u64 t = a;
+    a = b;
+    b = t;
+  }
+  switch (rule) {
+    case R_CALL: call(L, a, b); break;
+    case R_ERAS: L.rw[RW_ERAS]++; break;
+    case R_COMM22: comm22(L, a, b); break;
+    case R_ANNI: anni2(L, a, b); break;
+    case R_COMM02: comm02(L, a, b); break;
+    case R_OPNUM: op_num(L, a, b); break;
+    case R_SWITCHNUM: switch_num(L, a, b); break;
+    default: fail(L, ERR_UNIMPL); break;
+  }
 }
 
 // ---------------------------------------------------------------------------
 // lane setup / teardown
 // ---------------------------------------------------------------------------
-DEV void lane_init(Lane &L, const Params &P, u32 gwave, u32 lane, u64 *alloc_ctr) {
+DEV void lane_init(Lane &L, const Params &P, u32 gwave, u32 lane) {
   L.P = &P;
-  L.wave_node0 = P.ring_node0 + (u64)gwave * P.wave_nodes;
-  L.alloc_ctr = alloc_ctr;
+  L.sub_node0 = P.ring_node0 + (u64)gwave * P.wave_nodes + lane;
+  L.apos_home = &P.alloc_pos[(u64)gwave * 64 + lane];
+  L.apos = *L.apos_home;
+  L.slice_rest = 0;
+  L.ov_left = 0;
+  L.ov_cur = 0;
   L.spare = 0;
   L.dirty = false;
   L.T_lds = nullptr;
+  L.PL = nullptr;
   L.T_glb = P.trg_buf + (u64)gwave * P.trg_slots * 64 + lane;
   L.bag = nullptr;
   L.bag_top = nullptr;
-  L.bag_bot = 0;
+  L.bag_bot = nullptr;
 #pragma unroll
   for (u32 k = 0; k < RW_SLOTS; k++) L.rw[k] = 0;
 }
@@ -660,6 +816,7 @@ DEV void lane_init(Lane &L, const Params &P, u32 gwave, u32 lane, u64 *alloc_ctr
 DEV void lane_finish(Lane &L) {
   if (L.spare) st(*L.P, L.spare, P_FREE);
   L.spare = 0;
+  *L.apos_home = L.apos;
 }
 
 }  // namespace hvm

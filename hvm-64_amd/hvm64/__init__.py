@@ -240,9 +240,12 @@ class Net:
 
     def sched_stats(self):
         """Scheduler statistics of the last reduce launch (summed over waves)."""
-        out = (ctypes.c_uint64 * 7)()
+        out = (ctypes.c_uint64 * 20)()
         self.heap._check(self.heap._lib.hvm_gpu_sched_stats(self.heap._h, out))
-        keys = ["busy_iters", "idle_iters", "busy_ticks", "spills", "spilled", "claims", "claimed"]
+        keys = ["busy_iters", "idle_iters", "busy_ticks", "spills", "spilled", "claims", "claimed",
+                "interact_ticks", "spill_ticks", "alloc_miss", "ov_alloc",
+                "anni_iters", "comm_iters", "call_iters", "other_iters",
+                "anni_ticks", "comm_ticks", "call_ticks", "other_ticks"]
         return dict(zip(keys, [int(x) for x in out]))
 
     def last_kernel_ms(self):

@@ -133,10 +133,14 @@ int hvm_gpu_set_timeout(hvm_gpu *h, double seconds);
 int hvm_gpu_grid(hvm_gpu *h, uint32_t *blocks, uint32_t *waves);
 
 /* Scheduler statistics of the last reduction launch, summed over waves:
- * iterations with work, idle iterations, 100 MHz ticks spent in iterations
- * with work, spills to the global ring, redexes spilled, claims from the
- * global ring, redexes claimed. */
-int hvm_gpu_sched_stats(hvm_gpu *h, uint64_t out[7]);
+ * [0] iterations with work, [1] idle iterations, [2] 100 MHz ticks spent in
+ * iterations with work, [3] spills to the global ring, [4] redexes spilled,
+ * [5] claims from the global ring, [6] redexes claimed, [7] ticks inside the
+ * interaction rules, [8] ticks spilling; since the net was reset: [9] ring
+ * slots found in use on a later lap, [10] overflow-ring allocations;
+ * [11..14] iterations that ran anni / comm / call / other redexes,
+ * [15..18] ticks inside the rules for each of those classes; [19] 0. */
+int hvm_gpu_sched_stats(hvm_gpu *h, uint64_t out[20]);
 
 /* Host-only book utilities (no device).  Text results follow
  * hvm_gpu_readback's buffer convention; errors are described by
