@@ -115,6 +115,7 @@ void sync_globals(hvm_gpu *h) {
                       : e == ERR_TIMEOUT ? "reduction exceeded its time limit"
                       : e == ERR_BADPTR ? "invalid heap or book address (engine bug)"
                                           : "device error";
+    if (e == ERR_BADPTR) throw Error(code, std::string(msg) + ", check " + std::to_string(h->last.err_info));
     throw Error(code, msg);
   }
 }
@@ -128,7 +129,26 @@ void ensure_trg(hvm_gpu *h, u32 slots) {
   h->trg_slots = s;
 }
 
+// The device trusts the book's instructions (no per-step checks), so every
+// operand register index is checked here against its definition's size.
+void validate_book(const hvm_gpu *h) {
+  for (size_t i = 0; i < h->defs_host.size(); i++) {
+    const DefRec &d = h->defs_host[i];
+    if (d.nslots < 1 || (u64)d.ioff + d.ilen > h->instrs_host.size() / 2)
+      throw Error(HVM_E_ARG, "book: definition " + std::to_string(i) + " is malformed");
+    for (u32 k = 0; k < d.ilen; k++) {
+      u64 w0 = h->instrs_host[2 * ((u64)d.ioff + k)], w1 = h->instrs_host[2 * ((u64)d.ioff + k) + 1];
+      u32 kind = (u32)(w0 & 0xFF);
+      u32 a = (u32)(w0 >> 16) & 0xFFFF, b = (u32)(w0 >> 32) & 0xFFFF, c = (u32)(w0 >> 48);
+      u32 d4 = kind == I_WIRES ? (u32)(w1 & 0xFFFF) : 0;
+      if (kind > I_WIRES || a >= d.nslots || b >= d.nslots || c >= d.nslots || d4 >= d.nslots)
+        throw Error(HVM_E_ARG, "book: definition " + std::to_string(i) + " has an invalid instruction");
+    }
+  }
+}
+
 void upload_book(hvm_gpu *h) {
+  validate_book(h);
   dfree(h->d_defs);
   dfree(h->d_instrs);
   dfree(h->d_labbits);

@@ -38,6 +38,9 @@ enum : u32 {
   RW_SLOTS
 };
 
+// Which consistency check reported ERR_BADPTR (Globals::err_info).
+enum : u32 { BAD_HEAP_ADDR = 1, BAD_LINK_LIST, BAD_CHAIN, BAD_RESOLVE, BAD_SLOTS, BAD_DEF_CALL, BAD_DEF_EXPAND };
+
 // Scheduler statistics (summed over waves): iterations with work, idle
 // iterations, wall-clock ticks (100 MHz) spent in iterations with work,
 // spills to the global ring / redexes spilled, claims / redexes claimed.

@@ -49,10 +49,14 @@ __global__ void k_fill(u64 *words, u64 n, u64 value) {
 // (encode.rs:116-120, counts nothing).  Redexes go to the global ring.
 __global__ void k_exec(Params P, u32 def, u64 t0, u32 count_dref) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  __shared__ u64 s_pl[2 * LINK_CAP * 64];
+  __shared__ u64 s_t[LDS_SLOTS * 64];
   Lane L;
   lane_init(L, P, 0, 0);
+  L.PL = s_pl;
+  L.T_lds = s_t;
   if (count_dref) L.rw[RW_DREF]++;
-  interpret(L, def, t0);
+  run_task(L, false, 0, 0, def, t0);
   publish();
   lane_finish(L);
   for (u32 k = 0; k < RW_SLOTS; k++)
@@ -64,12 +68,16 @@ __global__ void k_exec(Params P, u32 def, u64 t0, u32 count_dref) {
 // to g->result.
 __global__ void k_expand(Params P, u64 old_root_cell) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  __shared__ u64 s_pl[2 * LINK_CAP * 64];
   Lane L;
   lane_init(L, P, 0, 0);
-  u64 r2 = alloc_var(L);
-  u64 e = alloc_var(L);
-  st(P, e, var_port(r2));
-  link(L, var_port(old_root_cell), expand_ref(e));
+  L.PL = s_pl;
+  u64 r2 = alloc_node(L), e = r2 + 8;
+  st2(P, r2, P_NONE, var_port(r2));
+  L.dirty = true;
+  u32 np = 0;
+  push_link(L, np, var_port(old_root_cell), expand_ref(e));
+  flush_links(L, np);
   publish();
   lane_finish(L);
   P.g->result = r2;
@@ -313,10 +321,8 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     if (lane == 0) top_c[c] = top - take;
     __builtin_amdgcn_wave_barrier();
     const u64 t_in = wall_clock64();
-    if (has) {
-      L.rw[RW_POPS]++;
-      interact(L, ra, rb);
-    }
+    if (has) L.rw[RW_POPS]++;
+    run_task(L, has, ra, rb, NO_DEF, 0);
     {
       const u64 dt = wall_clock64() - t_in;
       st[ST_INTERACT_TICKS] += dt;

@@ -14,6 +14,9 @@
 #define HVM_CAS(p, e, v) \
   __hip_atomic_compare_exchange_strong((p), (e), (v), __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
 #define HVM_LDS_FAA(p, v) atomicAdd((p), (v))
+// A pointer known to address LDS (ds_* instructions, never flat ones).
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+#define HVM_TO_LDS(p) ((lds_u64 *)(p))
 // Retire every outstanding vector memory operation of this wave (vmcnt(0)).
 #define HVM_PUBLISH() __builtin_amdgcn_s_waitcnt(0x0F70)
 #else
@@ -25,6 +28,8 @@
 #define HVM_FAA(p, v) __atomic_fetch_add((p), (v), __ATOMIC_RELAXED)
 #define HVM_CAS(p, e, v) __atomic_compare_exchange_n((p), (e), (v), false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)
 #define HVM_LDS_FAA(p, v) __atomic_fetch_add((p), (v), __ATOMIC_RELAXED)
+typedef unsigned long long lds_u64;
+#define HVM_TO_LDS(p) ((lds_u64 *)(p))
 #define HVM_PUBLISH() ((void)0)
 #endif
 
@@ -45,15 +50,16 @@ static const u32 LINK_CAP = 6;     // deferred links per lane (LDS)
 // ---------------------------------------------------------------------------
 // Every heap address is checked: a bad port (an engine bug) is redirected to
 // the sink words and recorded, so it can never touch memory outside the heap.
-NOINL void report_error(Globals *g, u32 code) {
+// Records the first error of a launch; `site` says which check found it.
+NOINL void report_error(Globals *g, u32 code, u32 site = 0) {
   u32 zero = 0;
-  HVM_CAS(&g->err, &zero, code);
+  if (HVM_CAS(&g->err, &zero, code)) HVM_ST(&g->err_info, site);
 }
 
 DEV u64 *hw(const Params &P, u64 addr) {
   u64 off = addr - HVM_HEAP_OFF;
   if (off >= P.heap_words * 8) {
-    report_error(P.g, ERR_BADPTR);
+    report_error(P.g, ERR_BADPTR, BAD_HEAP_ADDR);
     off = DUMP_WORD * 8;
   }
   return P.heap + (off >> 3);
@@ -85,7 +91,6 @@ struct Lane {
   u32 slice_rest;      // allocations left to take from the overflow ring
   u32 ov_left;         // nodes left in the current overflow run
   u64 ov_cur;          // next overflow-ring position of that run
-  u64 spare;           // reserved second word of a var pair, or 0
   bool dirty;          // stores of new nodes/cells not yet retired
   // links deferred by the running definition: pair i at PL[2i * 64],
   // PL[(2i + 1) * 64] (LDS; nullptr: link at once)
@@ -102,7 +107,7 @@ struct Lane {
   u32 rw[RW_SLOTS];
 };
 
-DEV void fail(Lane &L, u32 code) { report_error(L.P->g, code); }
+DEV void fail(Lane &L, u32 code, u32 site = 0) { report_error(L.P->g, code, site); }
 
 DEV u64 dump_addr() { return HVM_HEAP_OFF + DUMP_WORD * 8; }
 
@@ -196,25 +201,33 @@ DEV u64 alloc_node(Lane &L) {
   return c.addr;
 }
 
-// Var cells come in pairs from one node; both words are reserved at once.
-DEV u64 alloc_var(Lane &L) {
-  if (L.spare) {
-    u64 a = L.spare;
-    L.spare = 0;
-    return a;
-  }
+// Two node words.  Agent-scope stores: a plain store may stay in the
+// writing XCD's L2, where lanes on the other XCDs would not see it.
+DEV void st2(const Params &P, u64 addr, u64 w0, u64 w1) {
+  u64 *p = hw(P, addr);
+  HVM_ST(p, w0);
+  HVM_ST(p + 1, w1);
+}
+
+// A fresh pair of var cells (both words of one node): cells n and n + 8.
+DEV u64 alloc_vars2(Lane &L) {
   u64 n = alloc_node(L);
-  st(*L.P, n, P_NONE);
-  st(*L.P, n + 8, P_NONE);
+  st2(*L.P, n, P_NONE, P_NONE);
   L.dirty = true;
-  L.spare = n + 8;
+  return n;
+}
+
+// One fresh var cell; the node's other word stays FREE.
+DEV u64 alloc_var1(Lane &L) {
+  u64 n = alloc_node(L);
+  st2(*L.P, n, P_NONE, P_FREE);
+  L.dirty = true;
   return n;
 }
 
 DEV u64 new_node(Lane &L, u32 tag, u32 lab, u64 p1, u64 p2) {
   u64 n = alloc_node(L);
-  st(*L.P, n, norm_endpoint(p1));
-  st(*L.P, n + 8, norm_endpoint(p2));
+  st2(*L.P, n, norm_endpoint(p1), norm_endpoint(p2));
   L.dirty = true;
   return mkport(tag, lab, n);
 }
@@ -292,12 +305,28 @@ DEV void redux(Lane &L, u64 a, u64 b) {
 
 // Linking.  A link step either meets two principal ports (a redex) or swaps a
 // port into the var cell of an endpoint: the first endpoint to arrive leaves
-// its port there; the second finds it, frees the cell and links on.  Each
-// step consumes a var cell, so a longer chain means a corrupted net.
+// its port there; the second finds it, frees the cell and links on.
+//
+// Rules never link directly: they append (a, b) pairs to the lane's link
+// list (LDS, LINK_CAP pairs), and the lane's task loop (run_task) links the
+// whole list at one place in the code, four swaps in flight at a time.  A
+// swap that finds the other endpoint's port frees the cell and appends the
+// continuation link to the list.  Every continuation consumes a var cell, so
+// a longer run than MAX_CHAIN rounds means a corrupted net.
 static const u32 MAX_CHAIN = 1u << 24;
 
-// First step: returns true with `cell` set when a swap into `cell` of
-// norm_endpoint(b) is pending, false when the link is already complete.
+DEV void push_link(Lane &L, u32 &np, u64 a, u64 b) {
+  if (np >= LINK_CAP) {  // cannot happen: every producer checks the room it needs
+    fail(L, ERR_BADPTR, BAD_LINK_LIST);
+    return;
+  }
+  L.PL[(u64)(2 * np) * 64] = a;
+  L.PL[(u64)(2 * np + 1) * 64] = b;
+  np++;
+}
+
+// First step of one link: true with `cell` set when a swap of
+// norm_endpoint(b) into `cell` is due, false when the link is complete.
 DEV bool link_begin(Lane &L, u64 &a, u64 &b, u64 &cell) {
   if (!is_endpoint(a)) {
     if (!is_endpoint(b)) {
@@ -312,83 +341,42 @@ DEV bool link_begin(Lane &L, u64 &a, u64 &b, u64 &cell) {
   return true;
 }
 
-// Continues a link after its swap found the other endpoint's port `old` in
-// `cell`: the cell is ours to free, and `old` is linked on to `b`.
-DEV void link_more(Lane &L, u64 cell, u64 old, u64 b) {
+// Links every pair of the list (the list is empty afterwards).
+DEV void flush_links(Lane &L, u32 &np) {
   const Params &P = *L.P;
-  st(P, cell, P_FREE);
-  u64 a = old;
-  for (u32 hop = 0;; hop++) {
-    if (hop > MAX_CHAIN) {
-      fail(L, ERR_BADPTR);
+  u64 *PL = L.PL;
+  for (u32 round = 0; np > 0; round++) {
+    if (round > MAX_CHAIN) {
+      fail(L, ERR_BADPTR, BAD_CHAIN);
+      np = 0;
       return;
     }
-    u64 c;
-    if (!link_begin(L, a, b, c)) return;
-    publish_if_dirty(L);
-    u64 o = xchg(P, c, norm_endpoint(b));
-    if (o == P_NONE) return;
-    st(P, c, P_FREE);
-    a = o;
+    // take the last (up to) four pairs
+    const u32 cnt = np < 4 ? np : 4;
+    np -= cnt;
+    u64 a[4], b[4], c[4], o[4];
+    bool p[4];
+#pragma unroll
+    for (u32 j = 0; j < 4; j++) {
+      a[j] = j < cnt ? PL[(u64)(2 * (np + j)) * 64] : 0;
+      b[j] = j < cnt ? PL[(u64)(2 * (np + j) + 1) * 64] : 0;
+      c[j] = 0;
+    }
+#pragma unroll
+    for (u32 j = 0; j < 4; j++) p[j] = j < cnt && link_begin(L, a[j], b[j], c[j]);
+    if (p[0] || p[1] || p[2] || p[3]) publish_if_dirty(L);
+#pragma unroll
+    for (u32 j = 0; j < 4; j++) o[j] = p[j] ? xchg(P, c[j], norm_endpoint(b[j])) : P_NONE;
+#pragma unroll
+    for (u32 j = 0; j < 4; j++) {
+      if (o[j] != P_NONE) {  // second arrival: free the cell, link on
+        st(P, c[j], P_FREE);
+        PL[(u64)(2 * np) * 64] = o[j];
+        PL[(u64)(2 * np + 1) * 64] = b[j];
+        np++;
+      }
+    }
   }
-}
-
-DEV void link(Lane &L, u64 a, u64 b) {
-  u64 cell;
-  if (!link_begin(L, a, b, cell)) return;
-  publish_if_dirty(L);
-  u64 old = xchg(*L.P, cell, norm_endpoint(b));
-  if (old != P_NONE) link_more(L, cell, old, b);
-}
-
-// Two / four independent links whose first swaps are in flight together.
-DEV void link2(Lane &L, u64 a1, u64 b1, u64 a2, u64 b2) {
-  const Params &P = *L.P;
-  u64 c1 = 0, c2 = 0;
-  bool p1 = link_begin(L, a1, b1, c1);
-  bool p2 = link_begin(L, a2, b2, c2);
-  if (p1 || p2) publish_if_dirty(L);
-  u64 o1 = p1 ? xchg(P, c1, norm_endpoint(b1)) : P_NONE;
-  u64 o2 = p2 ? xchg(P, c2, norm_endpoint(b2)) : P_NONE;
-  if (o1 != P_NONE) link_more(L, c1, o1, b1);
-  if (o2 != P_NONE) link_more(L, c2, o2, b2);
-}
-
-DEV void link4(Lane &L, u64 a1, u64 b1, u64 a2, u64 b2, u64 a3, u64 b3, u64 a4, u64 b4) {
-  const Params &P = *L.P;
-  u64 c1 = 0, c2 = 0, c3 = 0, c4 = 0;
-  bool p1 = link_begin(L, a1, b1, c1);
-  bool p2 = link_begin(L, a2, b2, c2);
-  bool p3 = link_begin(L, a3, b3, c3);
-  bool p4 = link_begin(L, a4, b4, c4);
-  if (p1 || p2 || p3 || p4) publish_if_dirty(L);
-  u64 o1 = p1 ? xchg(P, c1, norm_endpoint(b1)) : P_NONE;
-  u64 o2 = p2 ? xchg(P, c2, norm_endpoint(b2)) : P_NONE;
-  u64 o3 = p3 ? xchg(P, c3, norm_endpoint(b3)) : P_NONE;
-  u64 o4 = p4 ? xchg(P, c4, norm_endpoint(b4)) : P_NONE;
-  if (o1 != P_NONE) link_more(L, c1, o1, b1);
-  if (o2 != P_NONE) link_more(L, c2, o2, b2);
-  if (o3 != P_NONE) link_more(L, c3, o3, b3);
-  if (o4 != P_NONE) link_more(L, c4, o4, b4);
-}
-
-// The first `cnt` of four links, their swaps in flight together.
-DEV void link_n(Lane &L, u32 cnt, u64 a1, u64 b1, u64 a2, u64 b2, u64 a3, u64 b3, u64 a4, u64 b4) {
-  const Params &P = *L.P;
-  u64 c1 = 0, c2 = 0, c3 = 0, c4 = 0;
-  bool p1 = link_begin(L, a1, b1, c1);
-  bool p2 = cnt > 1 && link_begin(L, a2, b2, c2);
-  bool p3 = cnt > 2 && link_begin(L, a3, b3, c3);
-  bool p4 = cnt > 3 && link_begin(L, a4, b4, c4);
-  if (p1 || p2 || p3 || p4) publish_if_dirty(L);
-  u64 o1 = p1 ? xchg(P, c1, norm_endpoint(b1)) : P_NONE;
-  u64 o2 = p2 ? xchg(P, c2, norm_endpoint(b2)) : P_NONE;
-  u64 o3 = p3 ? xchg(P, c3, norm_endpoint(b3)) : P_NONE;
-  u64 o4 = p4 ? xchg(P, c4, norm_endpoint(b4)) : P_NONE;
-  if (o1 != P_NONE) link_more(L, c1, o1, b1);
-  if (o2 != P_NONE) link_more(L, c2, o2, b2);
-  if (o3 != P_NONE) link_more(L, c3, o3, b3);
-  if (o4 != P_NONE) link_more(L, c4, o4, b4);
 }
 
 // Follow a wire endpoint whose other side has already linked (the
@@ -398,7 +386,7 @@ DEV u64 resolve(Lane &L, u64 p) {
   const Params &P = *L.P;
   for (u32 hop = 0; is_endpoint(p); hop++) {
     if (hop > MAX_CHAIN) {
-      fail(L, ERR_BADPTR);
+      fail(L, ERR_BADPTR, BAD_RESOLVE);
       return P_ERA;
     }
     u64 c = paddr(p);
@@ -413,53 +401,59 @@ DEV u64 resolve(Lane &L, u64 p) {
 DEV u64 as_wire(u64 p) { return is_endpoint(p) ? wire_port(paddr(p)) : p; }
 
 // ---------------------------------------------------------------------------
-// interaction rules (interact.rs)
+// interaction rules (interact.rs); each appends at most 4 links
 // ---------------------------------------------------------------------------
-DEV void anni2(Lane &L, u64 a, u64 b) {
+DEV void anni2(Lane &L, u32 &np, u64 a, u64 b) {
   L.rw[RW_ANNI]++;
   u64 a1, a2, b1, b2;
   take(L, a, a1, a2);
   take(L, b, b1, b2);
-  link2(L, a1, b1, a2, b2);
+  push_link(L, np, a1, b1);
+  push_link(L, np, a2, b2);
 }
 
-DEV void comm22(Lane &L, u64 a, u64 b) {
+DEV void comm22(Lane &L, u32 &np, u64 a, u64 b) {
   L.rw[RW_COMM]++;
   u64 a1, a2, b1, b2;
   take(L, a, a1, a2);
   take(L, b, b1, b2);
-  u64 v1 = alloc_var(L), v2 = alloc_var(L), v3 = alloc_var(L), v4 = alloc_var(L);
+  u64 v1 = alloc_vars2(L), v3 = alloc_vars2(L);
+  u64 v2 = v1 + 8, v4 = v3 + 8;
   u64 A1 = new_node(L, ptag(a), plab(a), var_port(v1), var_port(v2));
   u64 A2 = new_node(L, ptag(a), plab(a), var_port(v3), var_port(v4));
   u64 B1 = new_node(L, ptag(b), plab(b), var_port(v1), var_port(v3));
   u64 B2 = new_node(L, ptag(b), plab(b), var_port(v2), var_port(v4));
-  link4(L, a1, B1, a2, B2, b1, A1, b2, A2);
+  push_link(L, np, a1, B1);
+  push_link(L, np, a2, B2);
+  push_link(L, np, b1, A1);
+  push_link(L, np, b2, A2);
 }
 
-DEV void comm02(Lane &L, u64 a, u64 b) {
+DEV void comm02(Lane &L, u32 &np, u64 a, u64 b) {
   L.rw[RW_COMM]++;
   u64 b1, b2;
   take(L, b, b1, b2);
-  link2(L, b1, a, b2, a);
+  push_link(L, np, b1, a);
+  push_link(L, np, b2, a);
 }
 
-DEV void switch_num(Lane &L, u64 a, u64 b) {  // interact.rs:201-223
+DEV void switch_num(Lane &L, u32 &np, u64 a, u64 b) {  // interact.rs:201-223
   L.rw[RW_OPER]++;
   u64 a1, a2;
   take(L, a, a1, a2);
   u32 n = pnum(b);
   u32 v = num_tag(n) == N_U24 ? num_u24(n) : 0;
+  u64 x;
   if (v == 0) {
-    u64 x = new_node(L, T_CTR, 0, a2, P_ERA);
-    link(L, a1, x);
+    x = new_node(L, T_CTR, 0, a2, P_ERA);
   } else {
     u64 y = new_node(L, T_CTR, 0, numport(num_new_u24(v - 1)), a2);
-    u64 x = new_node(L, T_CTR, 0, P_ERA, y);
-    link(L, a1, x);
+    x = new_node(L, T_CTR, 0, P_ERA, y);
   }
+  push_link(L, np, a1, x);
 }
 
-DEV void op_num(Lane &L, u64 a, u64 b) {  // interact.rs:251-270
+DEV void op_num(Lane &L, u32 &np, u64 a, u64 b) {  // interact.rs:251-270
   u32 op = plab(a);
   u64 a1, a2;
   take(L, a, a1, a2);
@@ -469,10 +463,10 @@ DEV void op_num(Lane &L, u64 a, u64 b) {  // interact.rs:251-270
     bool div0 = false;
     u32 out = operate_binary(pnum(b), op, pnum(a1), &div0);
     if (div0) fail(L, ERR_DIV0);
-    link(L, a2, numport(out));
+    push_link(L, np, a2, numport(out));
   } else {
     u64 x = new_node(L, T_OP, N_SYM, numport(operate_unary(op, pnum(b))), a2);
-    link(L, a1, x);
+    push_link(L, np, a1, x);
   }
 }
 
@@ -486,193 +480,170 @@ DEV bool labs_has(const Params &P, const DefRec &d, u32 lab) {  // def.rs:30-37
   return (P.labbits[d.labs_off + idx] >> (lab & 63)) & 1;
 }
 
-
 // Definition instructions (def.rs:217-289, instruction.rs:390-493).
 //
-// A definition runs as a straight sequence of steps with the same shape for
-// every instruction kind, so that lanes running different definitions stay
-// converged on the memory operations: (1) the trg operand is read and, if
-// it is a wire, followed (Trg::target); (2) a CTR meeting a CTR of its own
-// label takes that node (the annihilation fast path of do_ctr); (3) the
-// kind's own work, which only allocates and stores; (4) the links it asks
-// for are appended to a per-lane list.  The list is linked four at a time
-// when it fills and when the definition ends.  Deferring a link inside a
-// definition changes nothing a later instruction can rely on: a trg read in
-// (1) sees either the port a link would have parked or the unresolved wire,
-// and both lead to the same net with the same rewrite counts (the fast
-// paths count what the interaction they replace counts).
-DEV void flush_links(Lane &L, u32 &np) {
-  u64 *PL = L.PL;
-  for (u32 i = 0; i < np; i += 4) {
-    u32 cnt = np - i < 4 ? np - i : 4;
-    u64 a[4], b[4];
-#pragma unroll
-    for (u32 j = 0; j < 4; j++) {
-      a[j] = j < cnt ? PL[(u64)(2 * (i + j)) * 64] : 0;
-      b[j] = j < cnt ? PL[(u64)(2 * (i + j) + 1) * 64] : 0;
-    }
-    link_n(L, cnt, a[0], b[0], a[1], b[1], a[2], b[2], a[3], b[3]);
-  }
-  np = 0;
-}
+// A definition runs as a sequence of steps with the same shape for every
+// instruction kind, so that lanes running different definitions stay
+// converged on the memory operations: (1) the trg operand is read and, if it
+// is a wire, followed (Trg::target); (2) a CTR meeting a CTR of its own label
+// takes that node (the annihilation fast path of do_ctr); (3) the kind's own
+// work, which only allocates and stores; (4) the link it asks for, if any,
+// is appended to the lane's link list.  The list is linked when it is full
+// and when the definition ends.  Deferring a link inside a definition changes
+// nothing a later instruction relies on: a trg read in (1) sees either the
+// port the link would have parked or the unresolved wire, and both lead to
+// the same net with the same rewrite counts (each fast path counts what the
+// interaction it replaces counts).
+struct Interp {
+  const u64 *ip;  // next instruction
+  lds_u64 *Tl;    // trg registers 0 .. LDS_SLOTS-1 (LDS, stride 64)
+  u64 *Tg;        // trg registers LDS_SLOTS .. (global, stride 64)
+  u32 left;       // instructions left; 0: no definition running
+};
 
-DEV void push_link(Lane &L, u32 &np, u64 a, u64 b) {
-  if (!L.PL) {
-    link(L, a, b);
-    return;
-  }
-  if (np == LINK_CAP) {  // list full: this one links at once
-    link(L, a, b);
-    return;
-  }
-  L.PL[(u64)(2 * np) * 64] = a;
-  L.PL[(u64)(2 * np + 1) * 64] = b;
-  np++;
+// Trg register s of this lane: LDS for s < LDS_SLOTS, else global memory
+// (explicit branches keep each access in its own address space).
+DEV u64 t_get(const Interp &I, u32 s) { return s < LDS_SLOTS ? I.Tl[s * 64] : I.Tg[(u64)s * 64]; }
+DEV void t_set(Interp &I, u32 s, u64 v) {
+  if (s < LDS_SLOTS) I.Tl[s * 64] = v;
+  else I.Tg[(u64)s * 64] = v;
 }
-
-#define T_AT(s) T[(u64)(s) * 64]
-DEV void exec_step(Lane &L, u64 *T, u32 lim, u64 w0, u64 w1, u32 &np) {
+#define T_AT(s) t_get(I, s)
+#define T_SET(s, v) t_set(I, s, v)
+DEV void exec_step(Lane &L, Interp &I, u64 w0, u64 w1, u32 &np) {
+  // operands (checked against the definition's size when the book loaded)
   const u32 kind = (u32)(w0 & 0xFF);
   const u32 a = (u32)(w0 >> 16) & 0xFFFF, b = (u32)(w0 >> 32) & 0xFFFF, c = (u32)(w0 >> 48);
-  const u32 d4 = kind == I_WIRES ? (u32)(w1 & 0xFFFF) : 0;
-  if (a >= lim || b >= lim || c >= lim || d4 >= lim || kind > I_WIRES) {
-    fail(L, kind > I_WIRES ? ERR_UNIMPL : ERR_BADPTR);
-    return;
-  }
   const u32 lab = (u32)(w1 & 0xFFFF);
-  // (1) operands
+  // (1) the trg operand, followed if it is a wire
   const u64 ta = (kind == I_CONST || kind == I_WIRES) ? 0 : T_AT(a);
-  const bool wants_port = kind >= I_CTR && kind <= I_SWITCH;
   u64 port = ta;
-  if (wants_port && ptag(ta) == T_RED) port = resolve(L, ta);
+  if (kind >= I_CTR && kind <= I_SWITCH && ptag(ta) == T_RED) port = resolve(L, ta);
   // (2) do_ctr's annihilation fast path (instruction.rs:406-410)
   const bool fast_ctr = kind == I_CTR && ptag(port) == T_CTR && plab(port) == lab;
   u64 x = 0, y = 0;
   if (fast_ctr) take(L, port, x, y);
   // (3) + (4)
-  switch (kind) {
-    case I_CONST: T_AT(a) = w1; break;
-    case I_LINK: push_link(L, np, ta, T_AT(b)); break;
-    case I_LINKCONST: push_link(L, np, ta, w1); break;
-    case I_CTR: {  // instruction.rs:403-422
-      if (fast_ctr) {
-        L.rw[RW_ANNI]++;
-        T_AT(b) = as_wire(x);
-        T_AT(c) = as_wire(y);
-        break;
-      }
-      u64 v1 = alloc_var(L), v2 = alloc_var(L);
-      push_link(L, np, port, new_node(L, T_CTR, lab, var_port(v1), var_port(v2)));
-      T_AT(b) = var_port(v1);
-      T_AT(c) = var_port(v2);
-      break;
+  if (kind == I_CTR) {  // instruction.rs:403-422
+    if (fast_ctr) {
+      L.rw[RW_ANNI]++;
+      T_SET(b, as_wire(x));
+      T_SET(c, as_wire(y));
+    } else {
+      u64 v = alloc_vars2(L);
+      push_link(L, np, port, new_node(L, T_CTR, lab, var_port(v), var_port(v + 8)));
+      T_SET(b, var_port(v));
+      T_SET(c, var_port(v + 8));
     }
-    case I_OP: {  // instruction.rs:425-442
-      if (ptag(port) == T_NUM) {
-        u64 v = alloc_var(L);
-        T_AT(b) = new_node(L, T_OP, N_SYM, numport(operate_unary(lab, pnum(port))), var_port(v));
-        T_AT(c) = var_port(v);
-      } else if (port == P_ERA) {
-        L.rw[SC_OP_ERA]++;
-        T_AT(b) = P_ERA;
-        T_AT(c) = P_ERA;
+  } else if (kind == I_LINK) {
+    push_link(L, np, ta, T_AT(b));
+  } else if (kind == I_LINKCONST) {
+    push_link(L, np, ta, w1);
+  } else if (kind == I_CONST) {
+    T_SET(a, w1);
+  } else if (kind == I_WIRES) {
+    const u32 d4 = (u32)(w1 & 0xFFFF);
+    u64 v = alloc_vars2(L);
+    T_SET(a, var_port(v));
+    T_SET(b, wire_port(v));
+    T_SET(c, var_port(v + 8));
+    T_SET(d4, wire_port(v + 8));
+  } else if (kind == I_OPNUM) {  // instruction.rs:445-462
+    const u32 rhs = (u32)(w1 >> 32);
+    if (ptag(port) == T_NUM) {
+      L.rw[RW_OPER]++;
+      bool div0 = false;
+      u32 out = operate_binary(pnum(port), lab, rhs, &div0);
+      if (div0) fail(L, ERR_DIV0);
+      T_SET(c, numport(out));
+    } else if (port == P_ERA) {
+      L.rw[SC_OPNUM_ERA]++;
+      T_SET(c, P_ERA);
+    } else {
+      u64 v = alloc_var1(L);
+      push_link(L, np, port, new_node(L, T_OP, lab, numport(rhs), var_port(v)));
+      T_SET(c, var_port(v));
+    }
+  } else if (kind == I_OP) {  // instruction.rs:425-442
+    if (ptag(port) == T_NUM) {
+      u64 v = alloc_var1(L);
+      T_SET(b, new_node(L, T_OP, N_SYM, numport(operate_unary(lab, pnum(port))), var_port(v)));
+      T_SET(c, var_port(v));
+    } else if (port == P_ERA) {
+      L.rw[SC_OP_ERA]++;
+      T_SET(b, P_ERA);
+      T_SET(c, P_ERA);
+    } else {
+      u64 v = alloc_vars2(L);
+      push_link(L, np, port, new_node(L, T_OP, lab, var_port(v), var_port(v + 8)));
+      T_SET(b, var_port(v));
+      T_SET(c, var_port(v + 8));
+    }
+  } else {  // I_SWITCH, instruction.rs:465-493
+    if (ptag(port) == T_NUM) {
+      L.rw[RW_OPER]++;
+      u32 n = pnum(port);
+      u32 v = num_tag(n) == N_U24 ? num_u24(n) : 0;
+      u64 w = alloc_var1(L);
+      if (v == 0) {
+        T_SET(b, new_node(L, T_CTR, 0, var_port(w), P_ERA));
       } else {
-        u64 v1 = alloc_var(L), v2 = alloc_var(L);
-        push_link(L, np, port, new_node(L, T_OP, lab, var_port(v1), var_port(v2)));
-        T_AT(b) = var_port(v1);
-        T_AT(c) = var_port(v2);
+        u64 c2 = new_node(L, T_CTR, 0, numport(num_new_u24(v - 1)), var_port(w));
+        T_SET(b, new_node(L, T_CTR, 0, P_ERA, c2));
       }
-      break;
-    }
-    case I_OPNUM: {  // instruction.rs:445-462
-      const u32 rhs = (u32)(w1 >> 32);
-      if (ptag(port) == T_NUM) {
-        L.rw[RW_OPER]++;
-        bool div0 = false;
-        u32 out = operate_binary(pnum(port), lab, rhs, &div0);
-        if (div0) fail(L, ERR_DIV0);
-        T_AT(c) = numport(out);
-      } else if (port == P_ERA) {
-        L.rw[SC_OPNUM_ERA]++;
-        T_AT(c) = P_ERA;
-      } else {
-        u64 v = alloc_var(L);
-        push_link(L, np, port, new_node(L, T_OP, lab, numport(rhs), var_port(v)));
-        T_AT(c) = var_port(v);
-      }
-      break;
-    }
-    case I_SWITCH: {  // instruction.rs:465-493
-      if (ptag(port) == T_NUM) {
-        L.rw[RW_OPER]++;
-        u32 n = pnum(port);
-        u32 v = num_tag(n) == N_U24 ? num_u24(n) : 0;
-        u64 w = alloc_var(L);
-        if (v == 0) {
-          T_AT(b) = new_node(L, T_CTR, 0, var_port(w), P_ERA);
-        } else {
-          u64 c2 = new_node(L, T_CTR, 0, numport(num_new_u24(v - 1)), var_port(w));
-          T_AT(b) = new_node(L, T_CTR, 0, P_ERA, c2);
-        }
-        T_AT(c) = wire_port(w);
-      } else if (port == P_ERA) {
-        L.rw[SC_SWITCH_ERA]++;
-        L.rw[RW_ERAS]++;
-        T_AT(b) = P_ERA;
-        T_AT(c) = P_ERA;
-      } else {
-        u64 v1 = alloc_var(L), v2 = alloc_var(L);
-        push_link(L, np, port, new_node(L, T_SWI, 0, var_port(v1), var_port(v2)));
-        T_AT(b) = var_port(v1);
-        T_AT(c) = var_port(v2);
-      }
-      break;
-    }
-    case I_WIRES: {
-      u64 v1 = alloc_var(L), v2 = alloc_var(L);
-      T_AT(a) = var_port(v1);
-      T_AT(b) = wire_port(v1);
-      T_AT(c) = var_port(v2);
-      T_AT(d4) = wire_port(v2);
-      break;
+      T_SET(c, wire_port(w));
+    } else if (port == P_ERA) {
+      L.rw[SC_SWITCH_ERA]++;
+      L.rw[RW_ERAS]++;
+      T_SET(b, P_ERA);
+      T_SET(c, P_ERA);
+    } else {
+      u64 v = alloc_vars2(L);
+      push_link(L, np, port, new_node(L, T_SWI, 0, var_port(v), var_port(v + 8)));
+      T_SET(b, var_port(v));
+      T_SET(c, var_port(v + 8));
     }
   }
 }
 #undef T_AT
+#undef T_SET
 
-// def.rs:217-289.  Instructions are fetched four at a time (one 64-byte
-// line); small definitions keep their trg registers in LDS.
-DEV void interpret(Lane &L, u32 di, u64 t0) {
+// def.rs:217-289: start definition `di` with t0 = `t0`.
+DEV void interp_begin(Lane &L, Interp &I, u32 di, u64 t0) {
   const Params &P = *L.P;
   const DefRec d = P.defs[di];
-  const bool in_lds = d.nslots <= LDS_SLOTS && L.T_lds;
-  u64 *T = in_lds ? L.T_lds : L.T_glb;
-  const u32 lim = in_lds ? LDS_SLOTS : P.trg_slots;
-  if (d.nslots > lim) {
-    fail(L, ERR_BADPTR);
+  I.Tl = HVM_TO_LDS(L.T_lds);
+  I.Tg = L.T_glb;
+  I.ip = P.instrs + (u64)d.ioff * 2;
+  I.left = d.ilen;
+  if (d.nslots > P.trg_slots) {  // the trg area is sized for the book
+    fail(L, ERR_BADPTR, BAD_SLOTS);
+    I.left = 0;
     return;
   }
-  T[0] = t0;
-  u32 np = 0;
-  const u64 *ip = P.instrs + (u64)d.ioff * 2;
-  for (u32 k = 0; k < d.ilen; k += 4) {
-    u64 w[8];
-#pragma unroll
-    for (u32 j = 0; j < 4; j++) {
-      bool in = k + j < d.ilen;
-      w[2 * j] = in ? ip[2 * (k + j)] : 0;
-      w[2 * j + 1] = in ? ip[2 * (k + j) + 1] : 0;
-    }
-    u32 cnt = d.ilen - k < 4 ? d.ilen - k : 4;
-    for (u32 j = 0; j < cnt; j++) exec_step(L, T, lim, w[2 * j], w[2 * j + 1], np);
+  I.Tl[0] = t0;
+}
+
+// Runs instructions until the definition ends or the link list is full
+// (every instruction appends at most one link).  Instructions are fetched
+// two at a time (32 bytes); exec_step has one call site (code size).
+DEV void interp_run(Lane &L, Interp &I, u32 &np) {
+  while (I.left > 0 && np < LINK_CAP) {
+    u32 n = I.left < 2 ? I.left : 2;
+    if (n > LINK_CAP - np) n = LINK_CAP - np;
+    const u64 w0 = I.ip[0], w1 = I.ip[1];
+    const u64 w2 = n > 1 ? I.ip[2] : 0, w3 = n > 1 ? I.ip[3] : 0;
+#pragma clang loop unroll(disable)
+    for (u32 j = 0; j < n; j++) exec_step(L, I, j ? w2 : w0, j ? w3 : w1, np);
+    I.ip += 2 * n;
+    I.left -= n;
   }
-  if (np) flush_links(L, np);
 }
 
 DEV u64 expand_ref(u64 cell) { return mkport(T_REF, 0xFFFF, cell); }
 
 // net.rs:353-380.  `cell` holds the ExpandDef's `out` endpoint.
-DEV void expand_call(Lane &L, u64 cell, u64 port) {
+DEV void expand_call(Lane &L, Interp &I, u32 &np, u64 cell, u64 port) {
   const Params &P = *L.P;
   if (ptag(port) == T_REF && port != P_ERA) {
     u64 oaddr = paddr(port);
@@ -680,61 +651,63 @@ DEV void expand_call(Lane &L, u64 cell, u64 port) {
       u64 x = ld(P, cell), y = ld(P, oaddr);
       st(P, cell, P_FREE);
       st(P, oaddr, P_FREE);
-      link(L, x, y);
+      push_link(L, np, x, y);
       return;
     }
     // net.call(port, Ref(self)): the book def meets this ExpandDef, which is
     // never a Ctr, so the call always dereferences (def.rs:182-188).
     if (oaddr == 0 || def_index(oaddr) >= P.ndefs) {
-      fail(L, ERR_BADPTR);
+      fail(L, ERR_BADPTR, BAD_DEF_EXPAND);
       return;
     }
     L.rw[RW_DREF]++;
-    interpret(L, def_index(oaddr), expand_ref(cell));
+    interp_begin(L, I, def_index(oaddr), expand_ref(cell));
     return;
   }
   u64 out = ld(P, cell);
   st(P, cell, P_FREE);
   u32 t = ptag(port);
   if (t == T_REF || t == T_NUM || t == T_VAR || t == T_RED) {
-    link(L, out, port);
+    push_link(L, np, out, port);
     return;
   }
   u64 p1, p2;
   take(L, port, p1, p2);
-  u64 v1 = alloc_var(L), v2 = alloc_var(L), e1 = alloc_var(L), e2 = alloc_var(L);
-  st(P, e1, var_port(v1));
-  st(P, e2, var_port(v2));
+  u64 v1 = alloc_vars2(L), e1 = alloc_node(L);
+  u64 v2 = v1 + 8, e2 = e1 + 8;
+  st2(P, e1, var_port(v1), var_port(v2));
+  L.dirty = true;
   u64 n = new_node(L, t, plab(port), var_port(v1), var_port(v2));
-  link(L, out, n);
-  link2(L, p1, expand_ref(e1), p2, expand_ref(e2));
+  push_link(L, np, out, n);
+  push_link(L, np, p1, expand_ref(e1));
+  push_link(L, np, p2, expand_ref(e2));
 }
 
 // def.rs:177-189
-DEV void call(Lane &L, u64 port, u64 trg) {
+DEV void call(Lane &L, Interp &I, u32 &np, u64 port, u64 trg) {
   const Params &P = *L.P;
   u64 addr = paddr(port);
   if (addr_in_heap(addr)) {  // ExpandDef: LabSet::ALL
     if (ptag(trg) == T_CTR && plab(trg) >= 0xFFFF) {
-      comm02(L, port, trg);
+      comm02(L, np, port, trg);
       return;
     }
     L.rw[RW_DREF]++;
-    expand_call(L, addr, trg);
+    expand_call(L, I, np, addr, trg);
     return;
   }
   u32 di = def_index(addr);
   if (addr == 0 || di >= P.ndefs) {
-    fail(L, ERR_BADPTR);
+    fail(L, ERR_BADPTR, BAD_DEF_CALL);
     return;
   }
   const DefRec &d = P.defs[di];
   if (ptag(trg) == T_CTR && !labs_has(P, d, plab(trg))) {
-    comm02(L, port, trg);
+    comm02(L, np, port, trg);
     return;
   }
   L.rw[RW_DREF]++;
-  interpret(L, di, trg);
+  interp_begin(L, I, di, trg);
 }
 
 // interact.rs:6-44.  The arms are tested in the reference's order; each arm
@@ -770,7 +743,7 @@ DEV u32 choose_rule(u64 a, u64 b, bool &swap) {
   return R_UNIMPL;  // Switch-Ctr(0), Op-Switch: unimplemented!() in the reference
 }
 
-DEV void interact(Lane &L, u64 a, u64 b) {
+DEV void interact(Lane &L, Interp &I, u32 &np, u64 a, u64 b) {
   bool swap;
   const u32 rule = choose_rule(a, b, swap);
   if (swap) {
@@ -779,14 +752,37 @@ DEV void interact(Lane &L, u64 a, u64 b) {
     b = t;
   }
   switch (rule) {
-    case R_CALL: call(L, a, b); break;
+    case R_CALL: call(L, I, np, a, b); break;
     case R_ERAS: L.rw[RW_ERAS]++; break;
-    case R_COMM22: comm22(L, a, b); break;
-    case R_ANNI: anni2(L, a, b); break;
-    case R_COMM02: comm02(L, a, b); break;
-    case R_OPNUM: op_num(L, a, b); break;
-    case R_SWITCHNUM: switch_num(L, a, b); break;
+    case R_COMM22: comm22(L, np, a, b); break;
+    case R_ANNI: anni2(L, np, a, b); break;
+    case R_COMM02: comm02(L, np, a, b); break;
+    case R_OPNUM: op_num(L, np, a, b); break;
+    case R_SWITCHNUM: switch_num(L, np, a, b); break;
     default: fail(L, ERR_UNIMPL); break;
+  }
+}
+
+// One lane's whole task: interact the redex (a, b) when `has`, else run
+// definition `di` with t0 (when di != NO_DEF), then link everything the task
+// asks for.  The loop alternates instructions and link rounds until the
+// definition (if any) has ended and the link list is empty; the link code is
+// shared by every rule and instruction.
+static const u32 NO_DEF = 0xFFFFFFFFu;
+
+DEV void run_task(Lane &L, bool has, u64 a, u64 b, u32 di, u64 t0) {
+  Interp I;
+  I.left = 0;
+  I.ip = nullptr;
+  I.Tl = nullptr;
+  I.Tg = nullptr;
+  u32 np = 0;
+  if (has) interact(L, I, np, a, b);
+  else if (di != NO_DEF) interp_begin(L, I, di, t0);
+  for (;;) {
+    if (I.left > 0) interp_run(L, I, np);
+    if (np > 0) flush_links(L, np);
+    if (I.left == 0) break;
   }
 }
 
@@ -801,7 +797,6 @@ DEV void lane_init(Lane &L, const Params &P, u32 gwave, u32 lane) {
   L.slice_rest = 0;
   L.ov_left = 0;
   L.ov_cur = 0;
-  L.spare = 0;
   L.dirty = false;
   L.T_lds = nullptr;
   L.PL = nullptr;
@@ -813,10 +808,6 @@ DEV void lane_init(Lane &L, const Params &P, u32 gwave, u32 lane) {
   for (u32 k = 0; k < RW_SLOTS; k++) L.rw[k] = 0;
 }
 
-DEV void lane_finish(Lane &L) {
-  if (L.spare) st(*L.P, L.spare, P_FREE);
-  L.spare = 0;
-  *L.apos_home = L.apos;
-}
+DEV void lane_finish(Lane &L) { *L.apos_home = L.apos; }
 
 }  // namespace hvm

@@ -46,11 +46,12 @@ class HvmError(RuntimeError):
 
 
 def load_library(path=None):
-    """Loads libhvm64_amd.so (raises if it has not been built)."""
+    """Loads libhvm64_amd.so (raises if it has not been built).  The
+    HVM64_LIB environment variable names another build of the library."""
     global _lib
     if _lib is not None:
         return _lib
-    path = path or LIB_PATH
+    path = path or os.environ.get("HVM64_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build it with hvm-64_amd/build.py (or __graft_entry__.build())")
     lib = ctypes.CDLL(path)
