@@ -94,11 +94,14 @@ __global__ void k_expand(Params P, u64 old_root_cell) {
 // oldest redexes of its fullest class) to the global ring while other waves
 // are idle; an empty wave claims up to 64 redexes from the global ring.
 // S = (queued << 32) | active waves; the reduction is complete when S == 0,
-// a state no wave can leave.  S is one cache line shared by every wave, so
-// busy waves touch it only to spill (and read it every 16 iterations), and
-// idle waves poll it with a growing, jittered backoff.
-static const u32 REFRESH = 16;      // busy iterations between control-word reads
-static const u32 MAX_BACKOFF = 1024;  // idle backoff cap, in s_sleep(2) units (~50 us)
+// a state no wave can leave.  S is one word shared by every wave and its
+// device-scope accesses all go to one memory channel, so busy waves touch it
+// only to spill (and read it every 16 iterations), and idle waves poll it
+// with a growing, jittered backoff capped near 1 ms: polling it faster slows
+// every other memory access of the reduction (measured: 2.6x slower overall
+// with a 50 us cap on bitonic sort of 2^16 keys).
+static const u32 REFRESH = 16;        // busy iterations between control-word reads
+static const u32 MAX_BACKOFF = 16384;  // idle backoff cap, in s_sleep(2) units (~0.9 ms)
 
 DEV void wave_sleep(u32 units) {
   for (u32 z = 0; z < units; z++) __builtin_amdgcn_s_sleep(2);
