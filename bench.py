@@ -67,6 +67,39 @@ def cpu_baseline(name, size, seed, seconds_budget=20.0):
             "single_thread_mips": round(mips_1, 2)}
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of `nbytes` (the HIP runtime's copy kernel,
+    reading and writing exactly nbytes) and one fill of `nbytes`: known-size
+    kernels that calibrate the FETCH_SIZE / WRITE_SIZE counters of a profile."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
+def recorded_traffic(name, size, seed):
+    """HBM bytes per k_reduce launch recorded by a rocprofv3 FETCH_SIZE /
+    WRITE_SIZE profile of this workload (profiles/k_reduce_*.md, scaled by
+    a known-size copy), or (None, None) when no profile covers it."""
+    import glob
+    key = f"{name}:{size}:{seed}"
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "k_reduce_*.md"))):
+        fields = {}
+        for line in open(path):
+            if " = " in line:
+                k, v = line.strip().split(" = ", 1)
+                fields[k] = v
+        if fields.get("workload_key") == key and "traffic_bytes_per_launch" in fields:
+            return int(fields["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def algorithmic_bytes_per_rewrite(name, size, seed):
     """Heap-word traffic of the reference algorithm per rewrite, counted by
     the metered oracle (8 B per load or store, 16 B per atomic RMW) on a
@@ -115,12 +148,7 @@ def main():
         return net
 
     if args.copy_mib:
-        import torch
-        src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        copy_calibration(args.copy_mib << 20)
 
     for _ in range(args.warmup):
         net = step()
@@ -143,9 +171,12 @@ def main():
     roofline = None
     if bpr is not None:
         achieved = bpr * rw.total() / (k_ms / 1e3) / 1e9
+        traffic, source = recorded_traffic(args.workload, size, args.seed)
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": traffic,
                     "bytes_per_rewrite": round(bpr, 2), "kernel": "k_reduce"}
+        if source:
+            roofline["traffic_source"] = source
     blocks, waves = heap.grid()
     out = {
         "metric": "MIPS on 1 MI355X: single net (bitonic sort)",
