@@ -115,12 +115,89 @@ def algorithmic_bytes_per_rewrite(name, size, seed):
         return None
 
 
+def bench_batch(args):
+    """Config 4: args.nets independent nets (tree-sum depth U[8,16] or bitonic
+    k U[6,12], seeded) reduced in one launch, against one launch sequence per
+    net (Net::new + boot + normal) on a sample of the same nets."""
+    import hvm64
+    from hvm64 import configs
+    book, nets, table = configs.batch_mix(args.nets, args.seed)
+    heap = hvm64.Heap(int(args.heap_gb * (1 << 30)))
+    heap.set_timeout(300)
+    host = hvm64.Host(book, heap)
+    defs = [host.defs[table[v][0]] for v in nets]
+    expected = [hvm64.u24_port(int(table[v][1])) for v in nets]
+
+    def batched():
+        b = hvm64.Batch(heap)
+        b.boot(defs)
+        b.normal()
+        return b
+
+    for _ in range(args.warmup):
+        batched()
+    t0 = time.perf_counter()
+    total = 0
+    kernel_ms = []
+    for _ in range(args.steps):
+        b = batched()
+        total += b.rwts.total()
+        kernel_ms.append(b.last_kernel_ms())
+    elapsed = time.perf_counter() - t0
+    ok = b.ports() == expected
+    rw = b.rwts
+    batched_mips = total / elapsed / 1e6
+
+    sample = min(args.per_net_sample, len(nets))
+    t1 = time.perf_counter()
+    per_total = 0
+    per_ok = True
+    for i in range(sample):
+        net = hvm64.Net(heap)
+        net.boot(defs[i])
+        net.normal()
+        per_total += net.rwts.total()
+    per_elapsed = time.perf_counter() - t1
+    per_net_mips = per_total / per_elapsed / 1e6
+    blocks, waves = heap.grid()
+    out = {
+        "metric": "MIPS on 1 MI355X: batched vs per-net launches",
+        "value": round(batched_mips, 2),
+        "unit": "MIPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded mix of in-net generated trees and keys)",
+        "config": {"workload": f"{args.nets} independent nets in one launch: tree-sum depth U[8,16] or bitonic "
+                               f"2^k keys k U[6,12] (seed {args.seed})",
+                   "nets": args.nets, "seed": args.seed, "heap_gb": args.heap_gb, "grid_blocks": blocks, "waves": waves},
+        "rewrites_per_step": rw.total(),
+        "rewrites": rw.as_dict(),
+        "kernel_ms": round(sum(kernel_ms) / len(kernel_ms), 3),
+        "result_ok": ok,
+        "per_net": {"mips": round(per_net_mips, 2), "nets_timed": sample, "rewrites": per_total,
+                    "seconds": round(per_elapsed, 3)},
+        "batched_over_per_net": round(batched_mips / per_net_mips, 2),
+    }
+    print(json.dumps(out), flush=True)
+    heap.close()
+    return 0 if ok else 1
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="bitonic", choices=["bitonic", "tree_sum", "church"])
+    ap.add_argument("--workload", default="bitonic", choices=["bitonic", "tree_sum", "church", "batch"])
+    ap.add_argument("--nets", type=int, default=4096, help="batch: number of nets")
+    ap.add_argument("--per-net-sample", type=int, default=256,
+                    help="batch: nets timed with one launch sequence each (the per-net baseline)")
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--heap-gb", type=float, default=32.0)
@@ -135,6 +212,8 @@ def main():
         return 2
 
     import hvm64
+    if args.workload == "batch":
+        return bench_batch(args)
     size, text, expected, desc = workload(args.workload, args.size, args.seed)
     heap = hvm64.Heap(int(args.heap_gb * (1 << 30)))
     heap.set_timeout(300)

@@ -17,6 +17,9 @@ hipError_t launch_fill(u64 *words, u64 n, u64 value, hipStream_t st);
 hipError_t launch_exec(const Params &P, u32 def, u64 t0, u32 count_dref, hipStream_t st);
 hipError_t launch_expand(const Params &P, u64 old_root_cell, hipStream_t st);
 hipError_t launch_reduce(const Params &P, u32 blocks, hipStream_t st);
+hipError_t launch_boot_batch(const Params &P, u32 n, const u32 *defs, u64 *roots, u32 max_blocks, hipStream_t st);
+hipError_t launch_expand_batch(const Params &P, u32 n, u64 *roots, u32 max_blocks, hipStream_t st);
+hipError_t launch_root_ports(const Params &P, u32 n, const u64 *roots, u64 *out, hipStream_t st);
 hipError_t launch_readback(const Params &P, u64 root_cell, u64 *stack, u64 stack_cap, u64 *out, u64 out_cap,
                            hipStream_t st);
 u32 reduce_waves_per_block();
@@ -43,6 +46,8 @@ struct hvm_gpu {
   u32 trg_slots = 0;
   u32 blocks = 0, waves = 0;
   u64 *d_rb_stack = nullptr, *d_rb_out = nullptr;
+  u64 *d_batch = nullptr;  // batch staging: roots, def ids, ports
+  u64 batch_cap = 0;       // words
   u64 rb_cap = 0;
   hvm64::Host host;
   std::vector<DefRec> defs_host;
@@ -200,7 +205,7 @@ void run_reduce(hvm_gpu *h, u64 limit) {
   h->last_ms = ms;
 }
 
-std::string readback(hvm_gpu *h) {
+std::string readback(hvm_gpu *h, u64 root_cell) {
   for (;;) {
     if (!h->d_rb_out) {
       h->rb_cap = std::max<u64>(h->rb_cap, 1 << 20);
@@ -208,7 +213,7 @@ std::string readback(hvm_gpu *h) {
       hip_check(hipMalloc(&h->d_rb_stack, h->rb_cap * sizeof(u64)), "hipMalloc readback");
     }
     Params P = make_params(h, ~0ull);
-    hip_check(launch_readback(P, h->root_cell, h->d_rb_stack, h->rb_cap, h->d_rb_out, h->rb_cap, h->stream),
+    hip_check(launch_readback(P, root_cell, h->d_rb_stack, h->rb_cap, h->d_rb_out, h->rb_cap, h->stream),
               "launch readback");
     hip_check(hipStreamSynchronize(h->stream), "readback");
     Globals g;
@@ -236,6 +241,16 @@ std::string readback(hvm_gpu *h) {
   }
 }
 
+// Device staging for batch calls: at least `words` 8-byte words.
+u64 *batch_buffer(hvm_gpu *h, u64 words) {
+  if (words > h->batch_cap) {
+    dfree(h->d_batch);
+    h->batch_cap = std::max<u64>(words, 1 << 16);
+    hip_check(hipMalloc(&h->d_batch, h->batch_cap * sizeof(u64)), "hipMalloc batch");
+  }
+  return h->d_batch;
+}
+
 template <class F>
 int guarded(hvm_gpu *h, F fn) {
   if (!h) return HVM_E_ARG;
@@ -256,6 +271,10 @@ int guarded(hvm_gpu *h, F fn) {
 extern "C" {
 
 int hvm_gpu_open(int device, uint64_t heap_bytes, hvm_gpu **out) {
+  return hvm_gpu_open_grid(device, heap_bytes, 0, out);
+}
+
+int hvm_gpu_open_grid(int device, uint64_t heap_bytes, uint32_t max_blocks, hvm_gpu **out) {
   if (!out) return HVM_E_ARG;
   *out = nullptr;
   auto h = std::make_unique<hvm_gpu>();
@@ -274,6 +293,7 @@ int hvm_gpu_open(int device, uint64_t heap_bytes, hvm_gpu **out) {
     hip_check(reduce_occupancy(&per_cu), "occupancy");
     if (per_cu < 1) per_cu = 1;
     h->blocks = (u32)(prop.multiProcessorCount * per_cu);
+    if (max_blocks && max_blocks < h->blocks) h->blocks = max_blocks;
     h->waves = h->blocks * reduce_waves_per_block();
     u64 words = heap_bytes / 8;
     u64 min_words = RESERVED_WORDS + (u64)h->waves * 64 * 2 * 8 + 4096;
@@ -323,6 +343,7 @@ void hvm_gpu_close(hvm_gpu *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   dfree(h->heap);
   dfree(h->alloc_pos);
+  dfree(h->d_batch);
   dfree(h->g);
   dfree(h->queue);
   dfree(h->d_defs);
@@ -467,6 +488,71 @@ int hvm_gpu_normal(hvm_gpu *h) {
   });
 }
 
+int hvm_gpu_boot_batch(hvm_gpu *h, uint32_t n, const uint32_t *def_ids, uint64_t *roots) {
+  return guarded(h, [&] {
+    if (n == 0) return;
+    if (!def_ids || !roots) throw Error(HVM_E_ARG, "null batch array");
+    for (u32 i = 0; i < n; i++)
+      if (def_ids[i] >= h->ndefs) throw Error(HVM_E_ARG, "no such definition in batch");
+    u64 *buf = batch_buffer(h, (u64)n + (n + 1) / 2);
+    u64 *d_roots = buf;
+    u32 *d_defs = (u32 *)(buf + n);
+    hip_check(hipMemcpyAsync(d_defs, def_ids, (size_t)n * 4, hipMemcpyHostToDevice, h->stream), "upload batch");
+    Params P = make_params(h, ~0ull);
+    hip_check(launch_boot_batch(P, n, d_defs, d_roots, h->waves, h->stream), "launch boot batch");
+    hip_check(hipMemcpyAsync(roots, d_roots, (size_t)n * 8, hipMemcpyDeviceToHost, h->stream), "download roots");
+    sync_globals(h);
+  });
+}
+
+int hvm_gpu_normal_batch(hvm_gpu *h, uint32_t n, uint64_t *roots) {
+  return guarded(h, [&] {
+    if (n && !roots) throw Error(HVM_E_ARG, "null batch array");
+    if (n) {
+      u64 *d_roots = batch_buffer(h, n);
+      hip_check(hipMemcpyAsync(d_roots, roots, (size_t)n * 8, hipMemcpyHostToDevice, h->stream), "upload roots");
+      Params P = make_params(h, ~0ull);
+      hip_check(launch_expand_batch(P, n, d_roots, h->waves, h->stream), "launch expand batch");
+      hip_check(hipMemcpyAsync(roots, d_roots, (size_t)n * 8, hipMemcpyDeviceToHost, h->stream), "download roots");
+      sync_globals(h);
+    }
+    run_reduce(h, ~0ull);
+  });
+}
+
+int hvm_gpu_root_ports(hvm_gpu *h, uint32_t n, const uint64_t *roots, uint64_t *ports) {
+  return guarded(h, [&] {
+    if (n == 0) return;
+    if (!roots || !ports) throw Error(HVM_E_ARG, "null batch array");
+    u64 *buf = batch_buffer(h, 2 * (u64)n);
+    hip_check(hipMemcpyAsync(buf, roots, (size_t)n * 8, hipMemcpyHostToDevice, h->stream), "upload roots");
+    Params P = make_params(h, ~0ull);
+    hip_check(launch_root_ports(P, n, buf, buf + n, h->stream), "launch root ports");
+    hip_check(hipMemcpyAsync(ports, buf + n, (size_t)n * 8, hipMemcpyDeviceToHost, h->stream), "download ports");
+    sync_globals(h);
+  });
+}
+
+int hvm_gpu_readback_root(hvm_gpu *h, uint64_t root, char *buf, uint64_t cap, uint64_t *len) {
+  int rc = 0;
+  int g = guarded(h, [&] {
+    if (root < HVM_HEAP_OFF || root >= HVM_HEAP_OFF + h->heap_words * 8) throw Error(HVM_E_ARG, "not a heap cell");
+    std::string text = readback(h, root);
+    if (len) *len = text.size();
+    if (!buf || cap < text.size() + 1) {
+      rc = HVM_E_TRUNC;
+      if (buf && cap) {
+        std::memcpy(buf, text.data(), cap - 1);
+        buf[cap - 1] = 0;
+      }
+      return;
+    }
+    std::memcpy(buf, text.data(), text.size());
+    buf[text.size()] = 0;
+  });
+  return g ? g : rc;
+}
+
 int hvm_gpu_reduce(hvm_gpu *h, uint64_t limit, uint64_t *popped, int *is_normal) {
   return guarded(h, [&] {
     u64 before = h->last.rw[RW_POPS];
@@ -496,7 +582,7 @@ int hvm_gpu_shortcuts(hvm_gpu *h, uint64_t out[3]) {
 int hvm_gpu_readback(hvm_gpu *h, char *buf, uint64_t cap, uint64_t *len) {
   int rc = 0;
   int g = guarded(h, [&] {
-    std::string text = readback(h);
+    std::string text = readback(h, h->root_cell);
     if (len) *len = text.size();
     if (!buf || cap < text.size() + 1) {
       rc = HVM_E_TRUNC;

@@ -63,6 +63,72 @@ __global__ void k_exec(Params P, u32 def, u64 t0, u32 count_dref) {
     if (L.rw[k]) __hip_atomic_fetch_add(&P.g->rw[k], (u64)L.rw[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A batch of independent nets: net i gets a fresh root cell (roots[i]) and
+// is booted with definition defs[i] (Net::boot per net, net.rs:30-32, one
+// DREF each).  One wave per block; each lane boots nets lane, lane + 64*B,
+// ...  Redexes go to the global ring.
+__global__ __launch_bounds__(64) void k_boot_batch(Params P, u32 n, const u32 *defs, u64 *roots) {
+  __shared__ u64 s_pl[2 * LINK_CAP * 64];
+  __shared__ u64 s_t[LDS_SLOTS * 64];
+  const u32 lane = threadIdx.x;
+  Lane L;
+  lane_init(L, P, blockIdx.x, lane);
+  L.PL = &s_pl[lane];
+  L.T_lds = &s_t[lane];
+  for (u32 i = blockIdx.x * 64 + lane; i < n; i += gridDim.x * 64) {
+    u32 di = defs[i];
+    if (di >= P.ndefs) {
+      fail(L, ERR_BADPTR, BAD_DEF_CALL);
+      roots[i] = 0;
+      continue;
+    }
+    u64 r = alloc_var1(L);
+    roots[i] = r;
+    L.rw[RW_DREF]++;
+    run_task(L, false, 0, 0, di, var_port(r));
+  }
+  publish();
+  lane_finish(L);
+  wave_flush_counters(L, lane);
+}
+
+// Net::expand for every net of a batch: roots[i] is replaced by the new root.
+__global__ __launch_bounds__(64) void k_expand_batch(Params P, u32 n, u64 *roots) {
+  __shared__ u64 s_pl[2 * LINK_CAP * 64];
+  const u32 lane = threadIdx.x;
+  Lane L;
+  lane_init(L, P, blockIdx.x, lane);
+  L.PL = &s_pl[lane];
+  for (u32 i = blockIdx.x * 64 + lane; i < n; i += gridDim.x * 64) {
+    u64 old = roots[i];
+    u64 r2 = alloc_node(L), e = r2 + 8;
+    st2(P, r2, P_NONE, var_port(r2));
+    L.dirty = true;
+    u32 np = 0;
+    push_link(L, np, var_port(old), expand_ref(e));
+    flush_links(L, np);
+    roots[i] = r2;
+  }
+  publish();
+  lane_finish(L);
+  wave_flush_counters(L, lane);
+}
+
+// The port each root cell leads to (following parked ports without
+// consuming them); a root whose wire is still open yields Var(root).
+__global__ void k_root_ports(Params P, u32 n, const u64 *roots, u64 *out) {
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    u64 cell = roots[i];
+    u64 p = var_port(cell);
+    for (u32 hop = 0; hop < MAX_CHAIN && is_endpoint(p); hop++) {
+      u64 v = ld(P, paddr(p));
+      if (v == P_NONE) break;
+      p = v;
+    }
+    out[i] = p;
+  }
+}
+
 // Net::expand (net.rs:65-69): a fresh root cell R2 and an ExpandDef whose
 // `out` is R2's other endpoint, linked to the old root.  Writes R2's address
 // to g->result.
@@ -463,6 +529,27 @@ hipError_t launch_exec(const Params &P, u32 def, u64 t0, u32 count_dref, hipStre
 }
 hipError_t launch_expand(const Params &P, u64 old_root_cell, hipStream_t st) {
   hipLaunchKernelGGL(k_expand, dim3(1), dim3(64), 0, st, P, old_root_cell);
+  return hipGetLastError();
+}
+hipError_t launch_boot_batch(const Params &P, u32 n, const u32 *defs, u64 *roots, u32 max_blocks, hipStream_t st) {
+  u32 blocks = (n + 63) / 64;
+  if (blocks > max_blocks) blocks = max_blocks;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_boot_batch, dim3(blocks), dim3(64), 0, st, P, n, defs, roots);
+  return hipGetLastError();
+}
+hipError_t launch_expand_batch(const Params &P, u32 n, u64 *roots, u32 max_blocks, hipStream_t st) {
+  u32 blocks = (n + 63) / 64;
+  if (blocks > max_blocks) blocks = max_blocks;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_expand_batch, dim3(blocks), dim3(64), 0, st, P, n, roots);
+  return hipGetLastError();
+}
+hipError_t launch_root_ports(const Params &P, u32 n, const u64 *roots, u64 *out, hipStream_t st) {
+  u32 blocks = (n + 255) / 256;
+  if (blocks == 0) blocks = 1;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_root_ports, dim3(blocks), dim3(256), 0, st, P, n, roots, out);
   return hipGetLastError();
 }
 hipError_t launch_reduce(const Params &P, u32 blocks, hipStream_t st) {

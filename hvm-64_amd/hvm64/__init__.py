@@ -31,7 +31,8 @@ EXPORTS = [
     "hvm_gpu_open", "hvm_gpu_close", "hvm_gpu_last_error", "hvm_gpu_load_book_text", "hvm_gpu_load_book",
     "hvm_gpu_def_id", "hvm_gpu_reset", "hvm_gpu_boot", "hvm_gpu_encode_net_text", "hvm_gpu_normal",
     "hvm_gpu_reduce", "hvm_gpu_rewrites", "hvm_gpu_shortcuts", "hvm_gpu_readback", "hvm_gpu_last_kernel_ms",
-    "hvm_gpu_grid", "hvm_gpu_set_timeout", "hvm_gpu_sched_stats",
+    "hvm_gpu_grid", "hvm_gpu_set_timeout", "hvm_gpu_sched_stats", "hvm_gpu_open_grid",
+    "hvm_gpu_boot_batch", "hvm_gpu_normal_batch", "hvm_gpu_root_ports", "hvm_gpu_readback_root",
     "hvm_book_print", "hvm_book_label_sets", "hvm_net_apply_tree",
 ]
 
@@ -76,6 +77,11 @@ def load_library(path=None):
     lib.hvm_gpu_grid.argtypes = [P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
     lib.hvm_gpu_set_timeout.argtypes = [P, ctypes.c_double]
     lib.hvm_gpu_sched_stats.argtypes = [P, ctypes.POINTER(u64)]
+    lib.hvm_gpu_open_grid.argtypes = [ctypes.c_int, u64, ctypes.c_uint32, ctypes.POINTER(P)]
+    lib.hvm_gpu_boot_batch.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(u64)]
+    lib.hvm_gpu_normal_batch.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(u64)]
+    lib.hvm_gpu_root_ports.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+    lib.hvm_gpu_readback_root.argtypes = [P, u64, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
     for name in ("hvm_book_print", "hvm_book_label_sets"):
         getattr(lib, name).argtypes = [ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
     lib.hvm_net_apply_tree.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
@@ -119,10 +125,12 @@ class Rewrites:
 class Heap:
     """Device heap + engine handle (Heap::new, runtime/src/allocator.rs:22)."""
 
-    def __init__(self, nbytes=1 << 30, device=0):
+    def __init__(self, nbytes=1 << 30, device=0, max_blocks=0):
+        """max_blocks caps the reduction grid (0: the whole device), so that
+        several heaps can reduce concurrently from separate host threads."""
         lib = load_library()
         h = ctypes.c_void_p()
-        rc = lib.hvm_gpu_open(device, nbytes, ctypes.byref(h))
+        rc = lib.hvm_gpu_open_grid(device, nbytes, max_blocks, ctypes.byref(h))
         if rc != 0:
             raise HvmError(rc, lib.hvm_gpu_last_error(None).decode())
         self._lib = lib
@@ -200,6 +208,20 @@ class Host:
             h._check(rc)
             return buf.value.decode()
 
+    def readback_root(self, root):
+        """Host::readback of one net of a Batch (its root cell)."""
+        h = self.heap
+        n = ctypes.c_uint64()
+        cap = 1 << 12
+        while True:
+            buf = ctypes.create_string_buffer(cap)
+            rc = h._lib.hvm_gpu_readback_root(h._h, root, buf, cap, ctypes.byref(n))
+            if rc == 9 and n.value + 1 > cap:
+                cap = n.value + 1
+                continue
+            h._check(rc)
+            return buf.value.decode()
+
     def encode_net(self, net, net_text):
         """Host::encode_net(net, Var(root), &ast) (host/src/encode.rs:116-120)."""
         h = self.heap
@@ -253,6 +275,43 @@ class Net:
         ms = ctypes.c_double()
         self.heap._check(self.heap._lib.hvm_gpu_last_kernel_ms(self.heap._h, ctypes.byref(ms)))
         return ms.value
+
+
+class Batch(Net):
+    """Independent nets reduced together in one launch: each net i behaves as
+    Net::new + Net::boot(defs[i]) + Net::normal() would on its own
+    (runtime/src/net.rs:19-60); rwts are the sums over the batch."""
+
+    def __init__(self, heap):
+        super().__init__(heap)
+        self.roots = []
+
+    def boot(self, defs):
+        """Boots one net per entry of `defs` (Host.defs items or indices)."""
+        n = len(defs)
+        ids = (ctypes.c_uint32 * n)(*[d.idx if hasattr(d, "idx") else int(d) for d in defs])
+        roots = (ctypes.c_uint64 * n)()
+        self.heap._check(self.heap._lib.hvm_gpu_boot_batch(self.heap._h, n, ids, roots))
+        self.roots = list(roots)
+
+    def normal(self):
+        n = len(self.roots)
+        roots = (ctypes.c_uint64 * n)(*self.roots)
+        self.heap._check(self.heap._lib.hvm_gpu_normal_batch(self.heap._h, n, roots))
+        self.roots = list(roots)
+
+    def ports(self):
+        """The port each net's root leads to (a Num port for numeric results)."""
+        n = len(self.roots)
+        roots = (ctypes.c_uint64 * n)(*self.roots)
+        out = (ctypes.c_uint64 * n)()
+        self.heap._check(self.heap._lib.hvm_gpu_root_ports(self.heap._h, n, roots, out))
+        return list(out)
+
+
+def u24_port(value):
+    """The Num port of a u24 value (runtime/src/port.rs:185-187, num/src/num.rs:86)."""
+    return ((value % (1 << 24)) << 8) | 3
 
 
 def _host_text(fn, *args):

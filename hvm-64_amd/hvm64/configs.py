@@ -11,8 +11,14 @@
 3. church(exp, labels): Church exponentiation c2 ^ c_exp applied to a Num
    increment on 0, every numeral a pure CTR/DUP net; the duplicators of c_exp
    cycle through `labels` distinct DUP labels.  Result 2**exp mod 2**24.
+4. batch_mix(n, seed): n independent nets, each a seeded choice of tree_sum
+   with depth uniform in [8, 16] or bitonic with k uniform in [6, 12], as one
+   book holding every (program, size) variant under its own name prefix.
+5. serving_sizes: the same mix with sizes log-uniform over those ranges.
 """
+import math
 import os
+import random
 import re
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -111,3 +117,58 @@ def church(exp=22, labels=16, base=2):
 
 def church_expected(exp=22, base=2):
     return str((base ** exp) % U24)
+
+
+_NAME = re.compile(r"@([A-Za-z0-9_.$]+)")
+
+
+def prefixed(book_text, prefix):
+    """The book with every definition name prefixed (refs included)."""
+    return _NAME.sub(lambda m: "@" + prefix + m.group(1), book_text)
+
+
+def variant(kind, size, seed=7):
+    """(book_text, expected_result) of one (program, size) variant."""
+    if kind == "tree_sum":
+        return tree_sum(size), tree_sum_expected(size)
+    if kind == "bitonic":
+        key_seed = (seed * 31 + size) % U24
+        return bitonic(size, key_seed), bitonic_expected(size, key_seed)
+    raise ValueError(kind)
+
+
+def variant_prefix(kind, size):
+    return f"{'t' if kind == 'tree_sum' else 'b'}{size}_"
+
+
+def mix_choice(rng, log_uniform=False):
+    """One net of the mix: a coin picks tree_sum (depth in [8, 16]) or bitonic
+    (k in [6, 12]); sizes are uniform, or log-uniform over the same ranges."""
+    kind = "tree_sum" if rng.random() < 0.5 else "bitonic"
+    lo, hi = (8, 16) if kind == "tree_sum" else (6, 12)
+    if log_uniform:
+        size = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+    else:
+        size = rng.randint(lo, hi)
+    return kind, size
+
+
+def mix_book(variants, seed=7):
+    """One book holding each (kind, size) variant under its prefix; returns
+    (book_text, {(kind, size): (main_name, expected)})."""
+    parts, table = [], {}
+    for kind, size in sorted(set(variants)):
+        text, expected = variant(kind, size, seed)
+        pre = variant_prefix(kind, size)
+        parts.append(prefixed(text, pre))
+        table[(kind, size)] = (pre + "main", expected)
+    return "\n".join(parts), table
+
+
+def batch_mix(n, seed=7, log_uniform=False):
+    """n nets of the batched workload: (book_text, nets, table) where nets[i]
+    = (kind, size) and table maps each variant to (main_name, expected)."""
+    rng = random.Random(seed)
+    nets = [mix_choice(rng, log_uniform) for _ in range(n)]
+    book, table = mix_book(nets, seed)
+    return book, nets, table

@@ -72,6 +72,12 @@ typedef struct {
  * `device`, allocates a heap of `heap_bytes` and an empty net. */
 int hvm_gpu_open(int device, uint64_t heap_bytes, hvm_gpu **out);
 
+/* hvm_gpu_open with the reduction grid capped at `max_blocks` blocks of 4
+ * waves (0: as many as fit on the device).  Handles opened with smaller
+ * grids can reduce concurrently on one device, each from its own host
+ * thread.  No counterpart in the reference (one Heap per Net there). */
+int hvm_gpu_open_grid(int device, uint64_t heap_bytes, uint32_t max_blocks, hvm_gpu **out);
+
 /* Frees all device memory of the handle. */
 void hvm_gpu_close(hvm_gpu *h);
 
@@ -110,6 +116,24 @@ int hvm_gpu_normal(hvm_gpu *h);
  * redexes.  *is_normal = 1 when no redex is left (Some(count)), else 0
  * (None); *popped = redexes popped. */
 int hvm_gpu_reduce(hvm_gpu *h, uint64_t limit, uint64_t *popped, int *is_normal);
+
+/* Batches of independent nets in one reduction.  No counterpart in the
+ * reference, which reduces one Net per Heap; each net behaves as its own
+ * Net::new + Net::boot + Net::normal (runtime/src/net.rs:19-60) would.
+ *
+ * hvm_gpu_boot_batch: nets 0..n-1 each get a fresh root cell, written to
+ *   roots[i], and are booted with definition def_ids[i] (one DREF each).
+ *   Nets booted after a reset share the handle's heap and counters.
+ * hvm_gpu_normal_batch: expands every root (roots[i] is replaced by the
+ *   expanded root) and reduces all booted nets to normal form in one launch.
+ *   hvm_gpu_rewrites then reports the sums over the batch.
+ * hvm_gpu_root_ports: the port each root leads to (a Num port for a numeric
+ *   result; Var(root) while the root's wire is open).
+ * hvm_gpu_readback_root: hvm_gpu_readback of one batch root. */
+int hvm_gpu_boot_batch(hvm_gpu *h, uint32_t n, const uint32_t *def_ids, uint64_t *roots);
+int hvm_gpu_normal_batch(hvm_gpu *h, uint32_t n, uint64_t *roots);
+int hvm_gpu_root_ports(hvm_gpu *h, uint32_t n, const uint64_t *roots, uint64_t *ports);
+int hvm_gpu_readback_root(hvm_gpu *h, uint64_t root, char *buf, uint64_t cap, uint64_t *len);
 
 /* ~ Net.rwts (runtime/src/runtime.rs:72-79): anni, comm, eras, dref, oper. */
 int hvm_gpu_rewrites(hvm_gpu *h, uint64_t out[5]);

@@ -203,3 +203,34 @@ def test_reference_panics_become_errors(gpu_heap, text, kind):
     assert e.value.kind == kind
     nf, rw, _ = run_gpu(gpu_heap, SNAPS["examples::machine_u32::num_add"]["text"])
     assert nf == "223"
+
+
+def test_batch_mix_against_oracle(gpu_heap, oracle):
+    """Config 4 (batched): 256 independent nets of the tree-sum / bitonic mix
+    in one launch.  Every net's result equals its expected value and the
+    oracle's normal form, and the batch's per-rule counts equal the sum of
+    the oracle's per-net counts (each net reduced alone)."""
+    import hvm64
+    from hvm64 import configs
+    book, nets, table = configs.batch_mix(256, seed=3)
+    print("case batch_mix 256 seed 3", flush=True)
+    host = hvm64.Host(book, gpu_heap)
+    batch = hvm64.Batch(gpu_heap)
+    batch.boot([host.defs[table[v][0]] for v in nets])
+    batch.normal()
+    ports = batch.ports()
+    ref = {}
+    for v in table:
+        text, expected = configs.variant(*v, seed=3)
+        r = oracle.run(text, heap_words=1 << 26)
+        assert r["nf"] == expected
+        ref[v] = r
+    for i, v in enumerate(nets):
+        assert ports[i] == hvm64.u24_port(int(table[v][1])), (i, v)
+    for i in range(0, 256, 37):
+        assert host.readback_root(batch.roots[i]) == ref[nets[i]]["nf"]
+    total = {k: 0 for k in ["rwts", "anni", "comm", "eras", "dref", "oper"]}
+    for v in nets:
+        for k in total:
+            total[k] += ref[v]["rwts"][k]
+    assert batch.rwts.as_dict() == total
