@@ -189,13 +189,65 @@ def bench_batch(args):
     return 0 if ok else 1
 
 
+def bench_serve(args):
+    """Config 5: seeded Poisson arrivals of the tree-sum / bitonic mix (sizes
+    log-uniform), continuous batching over `slots` concurrent heaps; reports
+    p50/p99 latency and aggregate MIPS."""
+    from hvm64 import serve
+    heap_bytes = int(args.heap_gb * (1 << 30))
+    srv = serve.Server(slots=args.slots, heap_bytes=heap_bytes, max_batch=args.max_batch, book_seed=args.seed)
+    try:
+        if args.warmup:
+            srv.run(serve.poisson_requests(args.rate, 0.2, seed=args.seed + 1))
+        reqs = serve.poisson_requests(args.rate, args.duration, seed=args.seed)
+        st = srv.run(reqs)
+    finally:
+        srv.close()
+    out = {
+        "metric": "p99 latency on 1 MI355X: serving (Poisson arrivals, continuous batching)",
+        "value": round(st.p99_ms, 2),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": args.warmup,
+        "ms_per_step": round(st.wall_s * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded Poisson arrivals of in-net generated trees and keys)",
+        "config": {"workload": f"Poisson arrivals at {args.rate:g}/s for {args.duration:g} s of tree-sum depth / "
+                               f"bitonic k log-uniform over [8,16] / [6,12] (seed {args.seed})",
+                   "rate": args.rate, "duration_s": args.duration, "slots": st.slots,
+                   "blocks_per_slot": st.per_slot_blocks, "heap_gb_per_slot": args.heap_gb,
+                   "max_batch": args.max_batch},
+        "p50_ms": round(st.p50_ms, 2),
+        "p99_ms": round(st.p99_ms, 2),
+        "max_ms": round(st.max_ms, 2),
+        "mips": round(st.mips, 2),
+        "requests": st.requests,
+        "completed": st.completed,
+        "result_ok": st.all_ok,
+        "batches": st.batches,
+        "mean_batch": round(st.mean_batch, 1),
+        "rewrites": st.rewrites,
+        "wall_s": round(st.wall_s, 3),
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if st.all_ok else 1
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="bitonic", choices=["bitonic", "tree_sum", "church", "batch"])
+    ap.add_argument("--workload", default="bitonic", choices=["bitonic", "tree_sum", "church", "batch", "serve"])
     ap.add_argument("--nets", type=int, default=4096, help="batch: number of nets")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serve: Poisson arrivals per second")
+    ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
+    ap.add_argument("--slots", type=int, default=4, help="serve: concurrent heaps (each with 1/slots of the grid)")
+    ap.add_argument("--max-batch", type=int, default=1024, help="serve: requests admitted per batch at most")
     ap.add_argument("--per-net-sample", type=int, default=256,
                     help="batch: nets timed with one launch sequence each (the per-net baseline)")
     ap.add_argument("--size", type=int, default=None)
@@ -214,6 +266,8 @@ def main():
     import hvm64
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     size, text, expected, desc = workload(args.workload, args.size, args.seed)
     heap = hvm64.Heap(int(args.heap_gb * (1 << 30)))
     heap.set_timeout(300)
