@@ -23,6 +23,7 @@ hipError_t launch_root_ports(const Params &P, u32 n, const u64 *roots, u64 *out,
 hipError_t launch_readback(const Params &P, u64 root_cell, u64 *stack, u64 stack_cap, u64 *out, u64 out_cap,
                            hipStream_t st);
 u32 reduce_waves_per_block();
+u32 reduce_book_lds_words();
 hipError_t reduce_occupancy(int *blocks_per_cu);
 }  // namespace hvm
 
@@ -95,6 +96,9 @@ Params make_params(hvm_gpu *h, u64 limit) {
   P.ndefs = h->ndefs;
   P.trg_slots = h->trg_slots;
   P.total_waves = h->waves;
+  P.instr_words = (u32)h->instrs_host.size();
+  P.lab_words = (u32)h->labbits_host.size();
+  P.book_lds = (u64)h->ndefs * 4 + P.instr_words + P.lab_words <= reduce_book_lds_words() ? 1 : 0;
   P.limit = limit;
   P.timeout_ticks = h->timeout_ticks;
   return P;

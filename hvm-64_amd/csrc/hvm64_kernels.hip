@@ -178,6 +178,9 @@ DEV u32 uniform(u32 v) { return __builtin_amdgcn_readfirstlane(v); }
 // Moves `cnt` redexes, the oldest of class `c`, from the wave's rings to the
 // global ring (all lanes; lane 0 updates the counters).  Returns S after.
 DEV u64 spill_to_global(const Params &P, u64 *bag, u32 *bot_c, u32 c, u32 cnt, u32 lane, u32 bot, u32 &err) {
+  // the nodes these redexes reach must be in memory before a claimer can
+  // read the ring slots (which it may do before S counts them)
+  publish();
   u64 base = 0;
   if (lane == 0) base = __hip_atomic_fetch_add(&P.g->q_tail, (u64)cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   base = __shfl(base, 0, 64);
@@ -202,7 +205,17 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   __shared__ u64 s_T[WAVES_PER_BLOCK][(LDS_SLOTS + 2 * LINK_CAP) * 64];
   __shared__ u32 s_top[WAVES_PER_BLOCK][NCLS];
   __shared__ u32 s_bot[WAVES_PER_BLOCK][NCLS];
+  __shared__ u64 s_book[BOOK_LDS_WORDS];
   const u32 wib = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the book, copied into LDS when it fits (the host decides: P.book_lds)
+  const u32 nd = P.ndefs * 4;
+  if (P.book_lds) {
+    const u64 *defs = (const u64 *)P.defs;
+    for (u32 i = threadIdx.x; i < nd; i += blockDim.x) s_book[i] = defs[i];
+    for (u32 i = threadIdx.x; i < P.instr_words; i += blockDim.x) s_book[nd + i] = P.instrs[i];
+    for (u32 i = threadIdx.x; i < P.lab_words; i += blockDim.x) s_book[nd + P.instr_words + i] = P.labbits[i];
+  }
+  __syncthreads();
   const u32 gwave = blockIdx.x * WAVES_PER_BLOCK + wib;
   Globals *g = P.g;
   u64 *bag = s_bag[wib];
@@ -217,6 +230,11 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   lane_init(L, P, gwave, lane);
   L.T_lds = &s_T[wib][lane];
   L.PL = &s_T[wib][LDS_SLOTS * 64 + lane];
+  if (P.book_lds) {
+    L.book = HVM_TO_LDS(s_book);
+    L.book_ins = nd;
+    L.book_lab = nd + P.instr_words;
+  }
   L.bag = bag;
   L.bag_top = top_c;
   L.bag_bot = bot_c;
@@ -411,8 +429,9 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
         err_next = gld32(&g->err);
       }
     }
-    publish();
-    L.dirty = false;
+    // No fence here: a node this wave created is visible to other waves only
+    // through a link (flush_links publishes before its swaps) or a spill
+    // (spill_to_global publishes before writing the global ring).
     __builtin_amdgcn_wave_barrier();
 
     // ---- clamp the rings after overflow pushes, then spill the excess
@@ -562,6 +581,7 @@ hipError_t launch_readback(const Params &P, u64 root_cell, u64 *stack, u64 stack
   return hipGetLastError();
 }
 u32 reduce_waves_per_block() { return WAVES_PER_BLOCK; }
+u32 reduce_book_lds_words() { return BOOK_LDS_WORDS; }
 hipError_t reduce_occupancy(int *blocks_per_cu) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_reduce, WAVES_PER_BLOCK * 64, 0);
 }

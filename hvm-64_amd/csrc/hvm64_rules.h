@@ -44,6 +44,9 @@ enum RedexClass : u32 { CLS_ANNI = 0, CLS_COMM = 1, CLS_CALL = 2, CLS_OTHER = 3,
 static const u32 CLS_CAP = 128;    // redexes per class ring (power of two)
 static const u32 LDS_SLOTS = 10;   // trg registers per lane kept in LDS
 static const u32 LINK_CAP = 6;     // deferred links per lane (LDS)
+// A book of at most BOOK_LDS_WORDS words (definition records, instructions,
+// label bits) is copied into every block's LDS by the reduce kernel.
+static const u32 BOOK_LDS_WORDS = 6144;
 
 // ---------------------------------------------------------------------------
 // heap access
@@ -103,6 +106,11 @@ struct Lane {
   u64 *bag;
   u32 *bag_top;
   const u32 *bag_bot;
+  // the book in LDS (nullptr: read it from global memory): definition
+  // records at 0 (4 words each), instructions at book_ins, label bits at
+  // book_lab
+  lds_u64 *book;
+  u32 book_ins, book_lab;
   // counters
   u32 rw[RW_SLOTS];
 };
@@ -473,11 +481,29 @@ DEV void op_num(Lane &L, u32 &np, u64 a, u64 b) {  // interact.rs:251-270
 // ---------------------------------------------------------------------------
 // definitions: call, instruction interpreter, ExpandDef
 // ---------------------------------------------------------------------------
-DEV bool labs_has(const Params &P, const DefRec &d, u32 lab) {  // def.rs:30-37
+// A definition record, from the LDS copy of the book when there is one.
+DEV DefRec load_def(const Lane &L, u32 di) {
+  if (L.book) {
+    DefRec d;
+    u64 w0 = L.book[4 * di], w1 = L.book[4 * di + 1], w2 = L.book[4 * di + 2];
+    d.ioff = (u32)w0;
+    d.ilen = (u32)(w0 >> 32);
+    d.nslots = (u32)w1;
+    d.labs_off = (u32)(w1 >> 32);
+    d.labs_words = (u32)w2;
+    d.min_safe = (u32)(w2 >> 32);
+    d.pad[0] = d.pad[1] = 0;
+    return d;
+  }
+  return L.P->defs[di];
+}
+
+DEV bool labs_has(const Lane &L, const DefRec &d, u32 lab) {  // def.rs:30-37
   if (lab >= d.min_safe) return false;
   u32 idx = lab >> 6;
   if (idx >= d.labs_words) return false;
-  return (P.labbits[d.labs_off + idx] >> (lab & 63)) & 1;
+  u64 w = L.book ? L.book[L.book_lab + d.labs_off + idx] : L.P->labbits[d.labs_off + idx];
+  return (w >> (lab & 63)) & 1;
 }
 
 // Definition instructions (def.rs:217-289, instruction.rs:390-493).
@@ -495,7 +521,8 @@ DEV bool labs_has(const Params &P, const DefRec &d, u32 lab) {  // def.rs:30-37
 // the same net with the same rewrite counts (each fast path counts what the
 // interaction it replaces counts).
 struct Interp {
-  const u64 *ip;  // next instruction
+  const u64 *ip;  // next instruction (global book)
+  u32 ipl;        // next instruction's word in the LDS book
   lds_u64 *Tl;    // trg registers 0 .. LDS_SLOTS-1 (LDS, stride 64)
   u64 *Tg;        // trg registers LDS_SLOTS .. (global, stride 64)
   u32 left;       // instructions left; 0: no definition running
@@ -611,10 +638,11 @@ DEV void exec_step(Lane &L, Interp &I, u64 w0, u64 w1, u32 &np) {
 // def.rs:217-289: start definition `di` with t0 = `t0`.
 DEV void interp_begin(Lane &L, Interp &I, u32 di, u64 t0) {
   const Params &P = *L.P;
-  const DefRec d = P.defs[di];
+  const DefRec d = load_def(L, di);
   I.Tl = HVM_TO_LDS(L.T_lds);
   I.Tg = L.T_glb;
   I.ip = P.instrs + (u64)d.ioff * 2;
+  I.ipl = L.book_ins + d.ioff * 2;
   I.left = d.ilen;
   if (d.nslots > P.trg_slots) {  // the trg area is sized for the book
     fail(L, ERR_BADPTR, BAD_SLOTS);
@@ -631,11 +659,26 @@ DEV void interp_run(Lane &L, Interp &I, u32 &np) {
   while (I.left > 0 && np < LINK_CAP) {
     u32 n = I.left < 2 ? I.left : 2;
     if (n > LINK_CAP - np) n = LINK_CAP - np;
-    const u64 w0 = I.ip[0], w1 = I.ip[1];
-    const u64 w2 = n > 1 ? I.ip[2] : 0, w3 = n > 1 ? I.ip[3] : 0;
+    u64 w0, w1, w2 = 0, w3 = 0;
+    if (L.book) {
+      w0 = L.book[I.ipl];
+      w1 = L.book[I.ipl + 1];
+      if (n > 1) {
+        w2 = L.book[I.ipl + 2];
+        w3 = L.book[I.ipl + 3];
+      }
+    } else {
+      w0 = I.ip[0];
+      w1 = I.ip[1];
+      if (n > 1) {
+        w2 = I.ip[2];
+        w3 = I.ip[3];
+      }
+    }
 #pragma clang loop unroll(disable)
     for (u32 j = 0; j < n; j++) exec_step(L, I, j ? w2 : w0, j ? w3 : w1, np);
     I.ip += 2 * n;
+    I.ipl += 2 * n;
     I.left -= n;
   }
 }
@@ -701,8 +744,8 @@ DEV void call(Lane &L, Interp &I, u32 &np, u64 port, u64 trg) {
     fail(L, ERR_BADPTR, BAD_DEF_CALL);
     return;
   }
-  const DefRec &d = P.defs[di];
-  if (ptag(trg) == T_CTR && !labs_has(P, d, plab(trg))) {
+  const DefRec d = load_def(L, di);
+  if (ptag(trg) == T_CTR && !labs_has(L, d, plab(trg))) {
     comm02(L, np, port, trg);
     return;
   }
@@ -774,6 +817,7 @@ DEV void run_task(Lane &L, bool has, u64 a, u64 b, u32 di, u64 t0) {
   Interp I;
   I.left = 0;
   I.ip = nullptr;
+  I.ipl = 0;
   I.Tl = nullptr;
   I.Tg = nullptr;
   u32 np = 0;
@@ -804,6 +848,9 @@ DEV void lane_init(Lane &L, const Params &P, u32 gwave, u32 lane) {
   L.bag = nullptr;
   L.bag_top = nullptr;
   L.bag_bot = nullptr;
+  L.book = nullptr;
+  L.book_ins = 0;
+  L.book_lab = 0;
 #pragma unroll
   for (u32 k = 0; k < RW_SLOTS; k++) L.rw[k] = 0;
 }
