@@ -24,6 +24,7 @@ hipError_t launch_readback(const Params &P, u64 root_cell, u64 *stack, u64 stack
                            hipStream_t st);
 u32 reduce_waves_per_block();
 u32 reduce_book_lds_words();
+u32 reduce_lds_slots();
 hipError_t reduce_occupancy(int *blocks_per_cu);
 }  // namespace hvm
 
@@ -45,6 +46,7 @@ struct hvm_gpu {
   u32 ndefs = 0;
   u64 *d_trg = nullptr;
   u32 trg_slots = 0;
+  u32 max_nslots = 0;   // widest definition of the loaded book
   u32 blocks = 0, waves = 0;
   u64 *d_rb_stack = nullptr, *d_rb_out = nullptr;
   u64 *d_batch = nullptr;  // batch staging: roots, def ids, ports
@@ -98,6 +100,7 @@ Params make_params(hvm_gpu *h, u64 limit) {
   P.total_waves = h->waves;
   P.instr_words = (u32)h->instrs_host.size();
   P.lab_words = (u32)h->labbits_host.size();
+  P.t_lds = h->max_nslots <= reduce_lds_slots() ? 1 : 0;
   P.book_lds = (u64)h->ndefs * 4 + P.instr_words + P.lab_words <= reduce_book_lds_words() ? 1 : 0;
   P.limit = limit;
   P.timeout_ticks = h->timeout_ticks;
@@ -172,6 +175,7 @@ void upload_book(hvm_gpu *h) {
   up(h->d_labbits, h->labbits_host);
   u32 slots = 1;
   for (auto &d : h->defs_host) slots = std::max(slots, d.nslots);
+  h->max_nslots = slots;
   ensure_trg(h, slots);
 }
 

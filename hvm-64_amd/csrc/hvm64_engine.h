@@ -103,7 +103,7 @@ struct Params {
   u32 book_lds;        // 1: the reduce kernel copies the book into LDS
   u32 instr_words;     // words in `instrs`
   u32 lab_words;       // words in `labbits`
-  u32 _pad;
+  u32 t_lds;           // 1: every definition's trg registers fit in LDS
   u64 limit;           // redex budget for reduce(limit); ~0 = unlimited
   u64 timeout_ticks;   // wall-clock (100 MHz) budget of a launch; 0 = none
 };

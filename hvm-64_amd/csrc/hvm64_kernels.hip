@@ -185,7 +185,7 @@ DEV u64 spill_to_global(const Params &P, u64 *bag, u32 *bot_c, u32 c, u32 cnt, u
   if (lane == 0) base = __hip_atomic_fetch_add(&P.g->q_tail, (u64)cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   base = __shfl(base, 0, 64);
   for (u32 i = lane; i < cnt; i += 64) {
-    u32 s2 = (c * CLS_CAP + ((bot + i) & (CLS_CAP - 1))) * 2;
+    u32 s2 = (ring_base(c) + ((bot + i) & (ring_cap(c) - 1))) * 2;
     u64 *q = P.queue + ((base + i) & P.qmask) * 2;
     gst(q, bag[s2]);
     gst(q + 1, bag[s2 + 1]);
@@ -201,7 +201,7 @@ DEV u64 spill_to_global(const Params &P, u64 *bag, u32 *bot_c, u32 c, u32 cnt, u
 }
 
 __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
-  __shared__ u64 s_bag[WAVES_PER_BLOCK][NCLS * CLS_CAP * 2];
+  __shared__ u64 s_bag[WAVES_PER_BLOCK][RING_PAIRS * 2];
   __shared__ u64 s_T[WAVES_PER_BLOCK][(LDS_SLOTS + 2 * LINK_CAP) * 64];
   __shared__ u32 s_top[WAVES_PER_BLOCK][NCLS];
   __shared__ u32 s_bot[WAVES_PER_BLOCK][NCLS];
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     u32 my_n = my_top - my_bot;
     u32 n = my_n;
 #pragma unroll
-    for (u32 o = 1; o < NCLS; o <<= 1) n += __shfl_xor(n, o, 64);
+    for (u32 o = 1; o < 16; o <<= 1) n += __shfl_xor(n, o, 64);
     n = uniform(n);
 
     // ---- empty rings: claim work from the global ring, or idle
@@ -339,7 +339,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
         // the rings are empty, so k <= 64 entries always fit
         u32 c = redex_class(a, b);
         u32 idx = HVM_LDS_FAA(&top_c[c], 1u);
-        u32 i = (c * CLS_CAP + (idx & (CLS_CAP - 1))) * 2;
+        u32 i = (ring_base(c) + (idx & (ring_cap(c) - 1))) * 2;
         bag[i] = a;
         bag[i + 1] = b;
       }
@@ -348,15 +348,15 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     }
 
     // ---- choose the fullest class (lowest index on ties)
-    u32 key = lane < NCLS ? (my_n << 2) | (NCLS - 1 - lane) : 0;
+    u32 key = lane < NCLS ? (my_n << 4) | (15 - lane) : 0;
 #pragma unroll
-    for (u32 o = 1; o < NCLS; o <<= 1) {
+    for (u32 o = 1; o < 16; o <<= 1) {
       u32 other = __shfl_xor(key, o, 64);
       key = other > key ? other : key;
     }
     key = uniform(key);
-    const u32 c = NCLS - 1 - (key & 3);
-    const u32 nc = key >> 2;
+    const u32 c = 15 - (key & 15);
+    const u32 nc = key >> 4;
     const u32 top = uniform(__shfl(my_top, c, 64));
     const u32 bot = uniform(__shfl(my_bot, c, 64));
 
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     bool has = lane < take;
     u64 ra = 0, rb = 0;
     if (has) {
-      u32 i = (c * CLS_CAP + ((top - 1 - lane) & (CLS_CAP - 1))) * 2;
+      u32 i = (ring_base(c) + ((top - 1 - lane) & (ring_cap(c) - 1))) * 2;
       ra = bag[i];
       rb = bag[i + 1];
     }
@@ -414,11 +414,12 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
       const u64 dt = wall_clock64() - t_in;
       st[ST_INTERACT_TICKS] += dt;
       // c is wave-uniform; unrolled so st stays in registers
+      const u32 sk = ring_stat(c);
 #pragma unroll
-      for (u32 cc = 0; cc < NCLS; cc++)
-        if (c == cc) {
-          st[ST_CLS_IT + cc]++;
-          st[ST_CLS_TICKS + cc] += dt;
+      for (u32 k = 0; k < 4; k++)
+        if (sk == k) {
+          st[ST_CLS_IT + k]++;
+          st[ST_CLS_TICKS + k] += dt;
         }
     }
     if (++busy >= REFRESH) {  // in flight while this iteration's stores retire
@@ -437,29 +438,29 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     // ---- clamp the rings after overflow pushes, then spill the excess
     my_top = lane < NCLS ? top_c[lane] : 0;
     my_bot = lane < NCLS ? bot_c[lane] : 0;
-    if (lane < NCLS && my_top - my_bot > CLS_CAP) {
-      my_top = my_bot + CLS_CAP;
+    if (lane < NCLS && my_top - my_bot > ring_cap(lane)) {
+      my_top = my_bot + ring_cap(lane);
       top_c[lane] = my_top;
     }
     my_n = my_top - my_bot;
     n = my_n;
 #pragma unroll
-    for (u32 o = 1; o < NCLS; o <<= 1) n += __shfl_xor(n, o, 64);
+    for (u32 o = 1; o < 16; o <<= 1) n += __shfl_xor(n, o, 64);
     n = uniform(n);
-    key = lane < NCLS ? (my_n << 2) | (NCLS - 1 - lane) : 0;
+    key = lane < NCLS ? (my_n << 4) | (15 - lane) : 0;
 #pragma unroll
-    for (u32 o = 1; o < NCLS; o <<= 1) {
+    for (u32 o = 1; o < 16; o <<= 1) {
       u32 other = __shfl_xor(key, o, 64);
       key = other > key ? other : key;
     }
     key = uniform(key);
-    const u32 sc = NCLS - 1 - (key & 3);
-    const u32 snc = key >> 2;
+    const u32 sc = 15 - (key & 15);
+    const u32 snc = key >> 4;
     u32 active_waves = (u32)(S_cache & 0xFFFFFFFFu);
     u32 idle_waves = P.total_waves > active_waves ? P.total_waves - active_waves : 0;
     u32 queued = (u32)(S_cache >> 32);
     u32 spill = 0;
-    if (snc > CLS_CAP * 3 / 4) spill = snc - CLS_CAP / 2;  // keep room for one iteration's pushes
+    if (snc > ring_cap(sc) * 3 / 4) spill = snc - ring_cap(sc) / 2;  // keep room for one iteration's pushes
     else if (n > 64 + 16 && idle_waves > 0 && queued < idle_waves * 64) spill = snc < n - 64 ? snc : n - 64;
     if (spill) {
       const u64 t_sp = wall_clock64();
@@ -582,6 +583,7 @@ hipError_t launch_readback(const Params &P, u64 root_cell, u64 *stack, u64 stack
 }
 u32 reduce_waves_per_block() { return WAVES_PER_BLOCK; }
 u32 reduce_book_lds_words() { return BOOK_LDS_WORDS; }
+u32 reduce_lds_slots() { return LDS_SLOTS; }
 hipError_t reduce_occupancy(int *blocks_per_cu) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_reduce, WAVES_PER_BLOCK * 64, 0);
 }

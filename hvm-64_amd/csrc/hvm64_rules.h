@@ -40,8 +40,16 @@ namespace hvm {
 static const u32 WAVES_PER_BLOCK = 4;
 // Every wave keeps its pending redexes in LDS, in one ring per rule class, so
 // that an iteration can run one rule on all 64 lanes.
-enum RedexClass : u32 { CLS_ANNI = 0, CLS_COMM = 1, CLS_CALL = 2, CLS_OTHER = 3, NCLS = 4 };
-static const u32 CLS_CAP = 128;    // redexes per class ring (power of two)
+// Definition calls go to NCALL rings by definition, so that the lanes of a
+// call iteration mostly run the same definition in lockstep.
+enum RedexClass : u32 { CLS_ANNI = 0, CLS_COMM = 1, CLS_OTHER = 2, CLS_CALL = 3, NCALL = 4, NCLS = CLS_CALL + NCALL };
+static const u32 CLS_CAP = 128;    // redexes per anni / comm / other ring (power of two)
+static const u32 CALL_CAP = 128;   // redexes per definition-call ring (power of two)
+static const u32 RING_PAIRS = CLS_CALL * CLS_CAP + NCALL * CALL_CAP;  // per wave
+DEV u32 ring_cap(u32 c) { return c < CLS_CALL ? CLS_CAP : CALL_CAP; }
+DEV u32 ring_base(u32 c) { return c < CLS_CALL ? c * CLS_CAP : CLS_CALL * CLS_CAP + (c - CLS_CALL) * CALL_CAP; }
+// Slot of a ring in the per-class statistics: anni, comm, call, other.
+DEV u32 ring_stat(u32 c) { return c == CLS_OTHER ? 3u : c >= CLS_CALL ? 2u : c; }
 static const u32 LDS_SLOTS = 10;   // trg registers per lane kept in LDS
 static const u32 LINK_CAP = 6;     // deferred links per lane (LDS)
 // A book of at most BOOK_LDS_WORDS words (definition records, instructions,
@@ -101,8 +109,8 @@ struct Lane {
   // trg registers: T[s * 64] (LDS for small definitions, else global)
   u64 *T_lds;
   u64 *T_glb;
-  // LDS redex rings, NCLS x CLS_CAP pairs (nullptr: push straight to the
-  // global ring); per-class push counters and oldest entries
+  // LDS redex rings, RING_PAIRS pairs (nullptr: push straight to the
+  // global ring); per-ring push counters and oldest entries
   u64 *bag;
   u32 *bag_top;
   const u32 *bag_bot;
@@ -278,10 +286,15 @@ DEV void global_push1(Lane &L, u64 a, u64 b) {
 }
 
 // The rule a redex will take, grouped: the arms of interact() that run
-// anni2, comm22, a definition call (a non-ERA Ref), and everything else.
+// anni2, comm22, a definition call (a non-ERA Ref; one ring per definition
+// modulo NCALL), and everything else.
 DEV u32 redex_class(u64 a, u64 b) {
   u32 at = ptag(a), bt = ptag(b);
-  if ((at == T_REF && a != P_ERA) || (bt == T_REF && b != P_ERA)) return CLS_CALL;
+  const bool aref = at == T_REF && a != P_ERA, bref = bt == T_REF && b != P_ERA;
+  if (aref || bref) {
+    const u64 addr = paddr(aref ? a : b);
+    return CLS_CALL + (addr_in_heap(addr) ? 0u : (u32)(addr >> 5) % NCALL);
+  }
   if (at == bt) {
     if (at == T_OP || at == T_SWI) return CLS_ANNI;
     if (at == T_CTR) return plab(a) == plab(b) ? CLS_ANNI : CLS_COMM;
@@ -301,8 +314,8 @@ DEV void redux(Lane &L, u64 a, u64 b) {
   if (L.bag) {
     u32 c = redex_class(a, b);
     u32 idx = HVM_LDS_FAA(&L.bag_top[c], 1u);
-    if (idx - L.bag_bot[c] < CLS_CAP) {
-      u32 k = (c * CLS_CAP + (idx & (CLS_CAP - 1))) * 2;
+    if (idx - L.bag_bot[c] < ring_cap(c)) {
+      u32 k = (ring_base(c) + (idx & (ring_cap(c) - 1))) * 2;
       L.bag[k] = a;
       L.bag[k + 1] = b;
       return;
@@ -323,11 +336,9 @@ DEV void redux(Lane &L, u64 a, u64 b) {
 // a longer run than MAX_CHAIN rounds means a corrupted net.
 static const u32 MAX_CHAIN = 1u << 24;
 
+// Every producer checks the room it needs (interp_run: one link per
+// instruction; a rule: at most four, into an empty list).
 DEV void push_link(Lane &L, u32 &np, u64 a, u64 b) {
-  if (np >= LINK_CAP) {  // cannot happen: every producer checks the room it needs
-    fail(L, ERR_BADPTR, BAD_LINK_LIST);
-    return;
-  }
   L.PL[(u64)(2 * np) * 64] = a;
   L.PL[(u64)(2 * np + 1) * 64] = b;
   np++;
@@ -526,114 +537,176 @@ struct Interp {
   lds_u64 *Tl;    // trg registers 0 .. LDS_SLOTS-1 (LDS, stride 64)
   u64 *Tg;        // trg registers LDS_SLOTS .. (global, stride 64)
   u32 left;       // instructions left; 0: no definition running
+  bool t_lds;     // every trg register is in LDS (else all in global memory)
 };
 
-// Trg register s of this lane: LDS for s < LDS_SLOTS, else global memory
-// (explicit branches keep each access in its own address space).
-DEV u64 t_get(const Interp &I, u32 s) { return s < LDS_SLOTS ? I.Tl[s * 64] : I.Tg[(u64)s * 64]; }
+// Trg register s of this lane: in LDS when every definition of the book
+// fits there (a kernel-wide choice, so the branch is uniform), else in the
+// lane's global trg area.
+DEV u64 t_get(const Interp &I, u32 s) { return I.t_lds ? I.Tl[s * 64] : I.Tg[(u64)s * 64]; }
 DEV void t_set(Interp &I, u32 s, u64 v) {
-  if (s < LDS_SLOTS) I.Tl[s * 64] = v;
+  if (I.t_lds) I.Tl[s * 64] = v;
   else I.Tg[(u64)s * 64] = v;
 }
-#define T_AT(s) t_get(I, s)
-#define T_SET(s, v) t_set(I, s, v)
-DEV void exec_step(Lane &L, Interp &I, u64 w0, u64 w1, u32 &np) {
-  // operands (checked against the definition's size when the book loaded)
-  const u32 kind = (u32)(w0 & 0xFF);
-  const u32 a = (u32)(w0 >> 16) & 0xFFFF, b = (u32)(w0 >> 32) & 0xFFFF, c = (u32)(w0 >> 48);
+
+// Two words of a node this lane has just allocated (always in the heap).
+DEV void st2_new(const Params &P, u64 addr, u64 w0, u64 w1) {
+  u64 *p = P.heap + ((addr - HVM_HEAP_OFF) >> 3);
+  HVM_ST(p, w0);
+  HVM_ST(p + 1, w1);
+}
+
+// Two fresh nodes: consecutive slice positions on the first lap.
+DEV void alloc2(Lane &L, u64 &n0, u64 &n1) {
+  if (L.apos + 2 <= L.P->wave_nodes / 64) {
+    n0 = HVM_HEAP_OFF + (L.sub_node0 + L.apos * 64) * 16;
+    n1 = n0 + 64 * 16;
+    L.apos += 2;
+    return;
+  }
+  n0 = alloc_node(L);
+  n1 = alloc_node(L);
+}
+
+// The rare instruction kinds (Op, OpNum, Switch; instruction.rs:425-493).
+DEV void exec_num(Lane &L, Interp &I, u32 kind, u32 b, u32 c, u64 w1, u64 port, u32 &np) {
+  const Params &P = *L.P;
   const u32 lab = (u32)(w1 & 0xFFFF);
-  // (1) the trg operand, followed if it is a wire
-  const u64 ta = (kind == I_CONST || kind == I_WIRES) ? 0 : T_AT(a);
-  u64 port = ta;
-  if (kind >= I_CTR && kind <= I_SWITCH && ptag(ta) == T_RED) port = resolve(L, ta);
-  // (2) do_ctr's annihilation fast path (instruction.rs:406-410)
-  const bool fast_ctr = kind == I_CTR && ptag(port) == T_CTR && plab(port) == lab;
-  u64 x = 0, y = 0;
-  if (fast_ctr) take(L, port, x, y);
-  // (3) + (4)
-  if (kind == I_CTR) {  // instruction.rs:403-422
-    if (fast_ctr) {
-      L.rw[RW_ANNI]++;
-      T_SET(b, as_wire(x));
-      T_SET(c, as_wire(y));
-    } else {
-      u64 v = alloc_vars2(L);
-      push_link(L, np, port, new_node(L, T_CTR, lab, var_port(v), var_port(v + 8)));
-      T_SET(b, var_port(v));
-      T_SET(c, var_port(v + 8));
-    }
-  } else if (kind == I_LINK) {
-    push_link(L, np, ta, T_AT(b));
-  } else if (kind == I_LINKCONST) {
-    push_link(L, np, ta, w1);
-  } else if (kind == I_CONST) {
-    T_SET(a, w1);
-  } else if (kind == I_WIRES) {
-    const u32 d4 = (u32)(w1 & 0xFFFF);
-    u64 v = alloc_vars2(L);
-    T_SET(a, var_port(v));
-    T_SET(b, wire_port(v));
-    T_SET(c, var_port(v + 8));
-    T_SET(d4, wire_port(v + 8));
-  } else if (kind == I_OPNUM) {  // instruction.rs:445-462
+  if (kind == I_OPNUM) {  // instruction.rs:445-462
     const u32 rhs = (u32)(w1 >> 32);
     if (ptag(port) == T_NUM) {
       L.rw[RW_OPER]++;
       bool div0 = false;
-      u32 out = operate_binary(pnum(port), lab, rhs, &div0);
+      const u32 out = operate_binary(pnum(port), lab, rhs, &div0);
       if (div0) fail(L, ERR_DIV0);
-      T_SET(c, numport(out));
+      t_set(I, c, numport(out));
     } else if (port == P_ERA) {
       L.rw[SC_OPNUM_ERA]++;
-      T_SET(c, P_ERA);
+      t_set(I, c, P_ERA);
     } else {
-      u64 v = alloc_var1(L);
-      push_link(L, np, port, new_node(L, T_OP, lab, numport(rhs), var_port(v)));
-      T_SET(c, var_port(v));
+      u64 v, n;
+      alloc2(L, v, n);
+      st2_new(P, v, P_NONE, P_FREE);
+      st2_new(P, n, numport(rhs), var_port(v));
+      L.dirty = true;
+      push_link(L, np, port, mkport(T_OP, lab, n));
+      t_set(I, c, var_port(v));
     }
-  } else if (kind == I_OP) {  // instruction.rs:425-442
+    return;
+  }
+  if (kind == I_OP) {  // instruction.rs:425-442
     if (ptag(port) == T_NUM) {
-      u64 v = alloc_var1(L);
-      T_SET(b, new_node(L, T_OP, N_SYM, numport(operate_unary(lab, pnum(port))), var_port(v)));
-      T_SET(c, var_port(v));
+      u64 v, n;
+      alloc2(L, v, n);
+      st2_new(P, v, P_NONE, P_FREE);
+      st2_new(P, n, numport(operate_unary(lab, pnum(port))), var_port(v));
+      L.dirty = true;
+      t_set(I, b, mkport(T_OP, N_SYM, n));
+      t_set(I, c, var_port(v));
     } else if (port == P_ERA) {
       L.rw[SC_OP_ERA]++;
-      T_SET(b, P_ERA);
-      T_SET(c, P_ERA);
+      t_set(I, b, P_ERA);
+      t_set(I, c, P_ERA);
     } else {
-      u64 v = alloc_vars2(L);
-      push_link(L, np, port, new_node(L, T_OP, lab, var_port(v), var_port(v + 8)));
-      T_SET(b, var_port(v));
-      T_SET(c, var_port(v + 8));
+      u64 v, n;
+      alloc2(L, v, n);
+      st2_new(P, v, P_NONE, P_NONE);
+      st2_new(P, n, var_port(v), var_port(v + 8));
+      L.dirty = true;
+      push_link(L, np, port, mkport(T_OP, lab, n));
+      t_set(I, b, var_port(v));
+      t_set(I, c, var_port(v + 8));
     }
-  } else {  // I_SWITCH, instruction.rs:465-493
-    if (ptag(port) == T_NUM) {
-      L.rw[RW_OPER]++;
-      u32 n = pnum(port);
-      u32 v = num_tag(n) == N_U24 ? num_u24(n) : 0;
-      u64 w = alloc_var1(L);
-      if (v == 0) {
-        T_SET(b, new_node(L, T_CTR, 0, var_port(w), P_ERA));
-      } else {
-        u64 c2 = new_node(L, T_CTR, 0, numport(num_new_u24(v - 1)), var_port(w));
-        T_SET(b, new_node(L, T_CTR, 0, P_ERA, c2));
-      }
-      T_SET(c, wire_port(w));
-    } else if (port == P_ERA) {
-      L.rw[SC_SWITCH_ERA]++;
-      L.rw[RW_ERAS]++;
-      T_SET(b, P_ERA);
-      T_SET(c, P_ERA);
+    return;
+  }
+  // I_SWITCH, instruction.rs:465-493
+  if (ptag(port) == T_NUM) {
+    L.rw[RW_OPER]++;
+    const u32 n = pnum(port);
+    const u32 v = num_tag(n) == N_U24 ? num_u24(n) : 0;
+    u64 w, x;
+    alloc2(L, w, x);
+    st2_new(P, w, P_NONE, P_FREE);
+    if (v == 0) {
+      st2_new(P, x, var_port(w), P_ERA);
+      t_set(I, b, mkport(T_CTR, 0, x));
     } else {
-      u64 v = alloc_vars2(L);
-      push_link(L, np, port, new_node(L, T_SWI, 0, var_port(v), var_port(v + 8)));
-      T_SET(b, var_port(v));
-      T_SET(c, var_port(v + 8));
+      const u64 y = alloc_node(L);
+      st2_new(P, x, numport(num_new_u24(v - 1)), var_port(w));
+      st2_new(P, y, P_ERA, mkport(T_CTR, 0, x));
+      t_set(I, b, mkport(T_CTR, 0, y));
     }
+    L.dirty = true;
+    t_set(I, c, wire_port(w));
+  } else if (port == P_ERA) {
+    L.rw[SC_SWITCH_ERA]++;
+    L.rw[RW_ERAS]++;
+    t_set(I, b, P_ERA);
+    t_set(I, c, P_ERA);
+  } else {
+    u64 v, m;
+    alloc2(L, v, m);
+    st2_new(P, v, P_NONE, P_NONE);
+    st2_new(P, m, var_port(v), var_port(v + 8));
+    L.dirty = true;
+    push_link(L, np, port, mkport(T_SWI, 0, m));
+    t_set(I, b, var_port(v));
+    t_set(I, c, var_port(v + 8));
   }
 }
-#undef T_AT
-#undef T_SET
+
+// One instruction (instruction.rs:390-493).  The common kinds come first
+// and take short paths: a wave executes the union of its lanes' paths, one
+// instruction every four cycles.
+DEV void exec_step(Lane &L, Interp &I, u64 w0, u64 w1, u32 &np) {
+  const Params &P = *L.P;
+  const u32 kind = (u32)(w0 & 0xFF);
+  const u32 a = (u32)(w0 >> 16) & 0xFFFF, b = (u32)(w0 >> 32) & 0xFFFF, c = (u32)(w0 >> 48);
+  if (kind == I_CONST) {
+    t_set(I, a, w1);
+    return;
+  }
+  if (kind == I_WIRES) {
+    u64 v, unused;
+    v = alloc_node(L);
+    (void)unused;
+    st2_new(P, v, P_NONE, P_NONE);
+    L.dirty = true;
+    t_set(I, a, var_port(v));
+    t_set(I, b, wire_port(v));
+    t_set(I, c, var_port(v + 8));
+    t_set(I, (u32)(w1 & 0xFFFF), wire_port(v + 8));
+    return;
+  }
+  const u64 ta = t_get(I, a);
+  if (kind == I_LINK || kind == I_LINKCONST) {
+    push_link(L, np, ta, kind == I_LINK ? t_get(I, b) : w1);
+    return;
+  }
+  // Ctr / Op / OpNum / Switch: the trg, followed if it is a wire
+  const u64 port = ptag(ta) == T_RED ? resolve(L, ta) : ta;
+  if (kind != I_CTR) {
+    exec_num(L, I, kind, b, c, w1, port, np);
+    return;
+  }
+  const u32 lab = (u32)(w1 & 0xFFFF);
+  if (ptag(port) == T_CTR && plab(port) == lab) {  // do_ctr's fast path (instruction.rs:406-410)
+    u64 x, y;
+    take(L, port, x, y);
+    L.rw[RW_ANNI]++;
+    t_set(I, b, as_wire(x));
+    t_set(I, c, as_wire(y));
+    return;
+  }
+  u64 v, n;  // instruction.rs:411-421
+  alloc2(L, v, n);
+  st2_new(P, v, P_NONE, P_NONE);
+  st2_new(P, n, var_port(v), var_port(v + 8));
+  L.dirty = true;
+  push_link(L, np, port, mkport(T_CTR, lab, n));
+  t_set(I, b, var_port(v));
+  t_set(I, c, var_port(v + 8));
+}
 
 // def.rs:217-289: start definition `di` with t0 = `t0`.
 DEV void interp_begin(Lane &L, Interp &I, u32 di, u64 t0) {
@@ -641,6 +714,7 @@ DEV void interp_begin(Lane &L, Interp &I, u32 di, u64 t0) {
   const DefRec d = load_def(L, di);
   I.Tl = HVM_TO_LDS(L.T_lds);
   I.Tg = L.T_glb;
+  I.t_lds = P.t_lds && L.T_lds;
   I.ip = P.instrs + (u64)d.ioff * 2;
   I.ipl = L.book_ins + d.ioff * 2;
   I.left = d.ilen;
@@ -649,7 +723,7 @@ DEV void interp_begin(Lane &L, Interp &I, u32 di, u64 t0) {
     I.left = 0;
     return;
   }
-  I.Tl[0] = t0;
+  t_set(I, 0, t0);
 }
 
 // Runs instructions until the definition ends or the link list is full
@@ -820,6 +894,7 @@ DEV void run_task(Lane &L, bool has, u64 a, u64 b, u32 di, u64 t0) {
   I.ipl = 0;
   I.Tl = nullptr;
   I.Tg = nullptr;
+  I.t_lds = false;
   u32 np = 0;
   if (has) interact(L, I, np, a, b);
   else if (di != NO_DEF) interp_begin(L, I, di, t0);
