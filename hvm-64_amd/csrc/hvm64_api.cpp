@@ -58,6 +58,8 @@ struct hvm_gpu {
   u64 root_cell = 0;
   u64 ring_node0 = 0, wave_nodes = 0, ov_node0 = 0, ov_nodes = 0;
   u64 *alloc_pos = nullptr;
+  u64 *idle_bits = nullptr, *mbox = nullptr;
+  u32 idle_words = 0;
   Globals last{};
   double last_ms = 0;
   u64 timeout_ticks = 0;  // wall-clock ticks a reduction may run (0 = unbounded)
@@ -100,6 +102,9 @@ Params make_params(hvm_gpu *h, u64 limit) {
   P.total_waves = h->waves;
   P.instr_words = (u32)h->instrs_host.size();
   P.lab_words = (u32)h->labbits_host.size();
+  P.idle_bits = h->idle_bits;
+  P.mbox = h->mbox;
+  P.idle_words = h->idle_words;
   P.t_lds = h->max_nslots <= reduce_lds_slots() ? 1 : 0;
   P.book_lds = (u64)h->ndefs * 4 + P.instr_words + P.lab_words <= reduce_book_lds_words() ? 1 : 0;
   P.limit = limit;
@@ -204,6 +209,8 @@ void run_reduce(hvm_gpu *h, u64 limit) {
   u64 zst[ST_SLOTS] = {};
   hip_check(hipMemcpyAsync(&h->g->done, &zero, sizeof(u64), hipMemcpyHostToDevice, h->stream), "budget");
   hip_check(hipMemcpyAsync(h->g->st, zst, sizeof(zst), hipMemcpyHostToDevice, h->stream), "stats");
+  hip_check(hipMemsetAsync(h->idle_bits, 0, (size_t)h->idle_words * 8, h->stream), "clear idle set");
+  hip_check(hipMemsetAsync(h->mbox, 0, (size_t)h->waves * MBOX_WORDS * 8, h->stream), "clear mailboxes");
   hip_check(hipEventRecord(h->ev0, h->stream), "event");
   hip_check(launch_reduce(P, h->blocks, h->stream), "launch reduce");
   hip_check(hipEventRecord(h->ev1, h->stream), "event");
@@ -324,6 +331,10 @@ int hvm_gpu_open_grid(int device, uint64_t heap_bytes, uint32_t max_blocks, hvm_
     if (ov < 64) throw std::runtime_error("heap too small for the grid");
     h->ov_nodes = ov;
     hip_check(hipMalloc(&h->alloc_pos, (size_t)h->waves * 64 * sizeof(u64)), "hipMalloc rings");
+    h->idle_words = (h->waves + 63) / 64;
+    if (h->idle_words > 64) throw Error(HVM_E_ARG, "grid too large");
+    hip_check(hipMalloc(&h->idle_bits, (size_t)h->idle_words * 8), "hipMalloc idle set");
+    hip_check(hipMalloc(&h->mbox, (size_t)h->waves * MBOX_WORDS * 8), "hipMalloc mailboxes");
     hip_check(hipMalloc(&h->g, sizeof(Globals)), "hipMalloc globals");
     // Global redex ring: 2^26 slots (1 GiB).  A massively parallel net keeps
     // about (lanes x recursion depth) redexes pending; the LDS rings hold
@@ -351,6 +362,8 @@ void hvm_gpu_close(hvm_gpu *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   dfree(h->heap);
   dfree(h->alloc_pos);
+  dfree(h->idle_bits);
+  dfree(h->mbox);
   dfree(h->d_batch);
   dfree(h->g);
   dfree(h->queue);

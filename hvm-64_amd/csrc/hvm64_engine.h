@@ -104,9 +104,18 @@ struct Params {
   u32 instr_words;     // words in `instrs`
   u32 lab_words;       // words in `labbits`
   u32 t_lds;           // 1: every definition's trg registers fit in LDS
+  u64 *idle_bits;      // one bit per wave of the reduce grid: the wave is idle
+  u64 *mbox;           // per-wave mailboxes (MBOX_WORDS each)
+  u32 idle_words;      // words in idle_bits
+  u32 _pad2;
   u64 limit;           // redex budget for reduce(limit); ~0 = unlimited
   u64 timeout_ticks;   // wall-clock (100 MHz) budget of a launch; 0 = none
 };
+
+// Mailbox of a wave: a count word on its own 128-byte line, then up to
+// MBOX_PAIRS redex pairs handed to the wave by another wave.
+static const u32 MBOX_PAIRS = 64;
+static const u32 MBOX_WORDS = 16 + 2 * MBOX_PAIRS;
 
 // Heap layout: the first RESERVED_WORDS words are fixed cells.
 static const u64 RESERVED_WORDS = 64;

@@ -168,6 +168,9 @@ __global__ void k_expand(Params P, u64 old_root_cell) {
 // with a 50 us cap on bitonic sort of 2^16 keys).
 static const u32 REFRESH = 16;        // busy iterations between control-word reads
 static const u32 MAX_BACKOFF = 16384;  // idle backoff cap, in s_sleep(2) units (~0.9 ms)
+static const u32 MBOX_POLL = 16;       // s_sleep(2) units between mailbox polls (~1 us)
+static const u32 IDLE_CHECK = 64;      // mailbox polls between reads of S (a power of two)
+static const u32 MBOX_KEEP = 80;       // redexes a busy wave keeps before handing some off
 
 DEV void wave_sleep(u32 units) {
   for (u32 z = 0; z < units; z++) __builtin_amdgcn_s_sleep(2);
@@ -249,6 +252,17 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   u32 err_next = 0;
   bool fresh = false;          // S_next / err_next hold a read to consume
   u64 st[ST_SLOTS] = {};
+  // mailboxes: off under a reduce(limit) budget (a wave that stops there
+  // must not have a delivery in flight)
+  const bool use_mbox = !limited && P.mbox != nullptr && P.idle_bits != nullptr;
+  u64 *my_box = use_mbox ? P.mbox + (u64)gwave * MBOX_WORDS : nullptr;
+  bool in_set = false;         // this wave's bit is set in P.idle_bits
+  u32 idle_polls = 0;
+  if (use_mbox) {  // every wave starts idle: in the idle set, not counted in S
+    if (lane == 0)
+      __hip_atomic_fetch_or(&P.idle_bits[gwave >> 6], 1ull << (gwave & 63), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    in_set = true;
+  }
 
   for (;;) {
     if (fresh) {
@@ -269,81 +283,140 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     for (u32 o = 1; o < 16; o <<= 1) n += __shfl_xor(n, o, 64);
     n = uniform(n);
 
-    // ---- empty rings: claim work from the global ring, or idle
+    // ---- empty rings: idle.  An idle wave is in the idle set (its bit in
+    // P.idle_bits) and polls its own mailbox, where a busy wave can hand it
+    // redexes directly; every IDLE_CHECK polls it also reads S for the end
+    // of the reduction and for redexes on the global ring.
     if (n == 0) {
-      u32 k = 0;
-      u64 s = 0;
-      u32 err = 0;
-      if (lane == 0) {
-        s = gld(&g->S);
-        err = gld32(&g->err);
-        for (int tries = 0; tries < 4 && !err; tries++) {
-          u64 q = s >> 32;
-          if (q == 0) break;
-          u64 want = q < 64 ? q : 64;
-          u64 ns = s - (want << 32) + (active ? 0 : 1);
-          if (__hip_atomic_compare_exchange_strong(&g->S, &s, ns, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT)) {
-            k = (u32)want;
-            s = ns;
-            break;
+      if (active) {
+        if (use_mbox && lane == 0)
+          __hip_atomic_fetch_or(&P.idle_bits[gwave >> 6], 1ull << (gwave & 63), __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+        in_set = use_mbox;
+        u64 old = 0;
+        if (lane == 0) old = __hip_atomic_fetch_sub(&g->S, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        S_cache = __shfl(old, 0, 64) - 1;
+        active = false;
+        idle_polls = 0;
+        if (S_cache == 0) break;  // nothing active, nothing queued: done
+      }
+      st[ST_IDLE_IT]++;
+      if (use_mbox) {
+        u64 k = 0;
+        if (lane == 0) k = gld(my_box);
+        k = __shfl(k, 0, 64);
+        if (k > 0) {  // a delivery: the sender already counted this wave active
+          if (lane < k) {
+            const u64 a = gld(my_box + 16 + 2 * lane), b = gld(my_box + 16 + 2 * lane + 1);
+            const u32 c = redex_class(a, b);  // the rings are empty: k <= 64 entries fit
+            const u32 idx = HVM_LDS_FAA(&top_c[c], 1u);
+            const u32 i = (ring_base(c) + (idx & (ring_cap(c) - 1))) * 2;
+            bag[i] = a;
+            bag[i + 1] = b;
           }
+          if (lane == 0) gst(my_box, 0);
+          st[ST_CLAIMS]++;
+          st[ST_CLAIMED] += k;
+          active = true;
+          in_set = false;
+          backoff = 1 + (gwave & 7);
+          __builtin_amdgcn_wave_barrier();
+          continue;
         }
       }
-      k = __shfl(k, 0, 64);
-      s = __shfl(s, 0, 64);
-      err = __shfl(err, 0, 64);
-      if (err) break;
-      S_cache = s;
-      if (k == 0) {
-        st[ST_IDLE_IT]++;
-        if (active) {
-          u64 old = 0;
-          if (lane == 0) old = __hip_atomic_fetch_sub(&g->S, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          s = __shfl(old, 0, 64) - 1;
-          active = false;
+      if (!use_mbox || (idle_polls++ & (IDLE_CHECK - 1)) == 0) {
+        u64 s = 0;
+        u32 err = 0;
+        if (lane == 0) {
+          s = gld(&g->S);
+          err = gld32(&g->err);
         }
+        s = __shfl(s, 0, 64);
+        err = __shfl(err, 0, 64);
+        S_cache = s;
+        if (err) break;
         if (s == 0) break;
         if (limited && gld(&g->done) >= P.limit) break;
         if (wall_clock64() > t_end) {
           fail(L, ERR_TIMEOUT);
           break;
         }
+        // redexes on the global ring: leave the idle set, then claim
+        if ((s >> 32) > 0 && (!use_mbox || in_set)) {
+          bool left = true;
+          if (use_mbox) {
+            u64 old = 0;
+            if (lane == 0)
+              old = __hip_atomic_fetch_and(&P.idle_bits[gwave >> 6], ~(1ull << (gwave & 63)), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            left = ((__shfl(old, 0, 64) >> (gwave & 63)) & 1) != 0;
+            in_set = false;  // either we left, or a sender took us out and a delivery is coming
+          }
+          if (left) {
+            u32 k = 0;
+            if (lane == 0) {
+              for (int tries = 0; tries < 4; tries++) {
+                u64 q = s >> 32;
+                if (q == 0) break;
+                u64 want = q < 64 ? q : 64;
+                u64 ns = s - (want << 32) + 1;
+                if (__hip_atomic_compare_exchange_strong(&g->S, &s, ns, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)) {
+                  k = (u32)want;
+                  break;
+                }
+              }
+            }
+            k = __shfl(k, 0, 64);
+            if (k > 0) {
+              backoff = 1 + (gwave & 7);
+              active = true;
+              st[ST_CLAIMS]++;
+              st[ST_CLAIMED] += k;
+              u64 base = 0;
+              if (lane == 0) base = __hip_atomic_fetch_add(&g->q_head, (u64)k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              base = __shfl(base, 0, 64);
+              if (lane < k) {
+                u64 *slot = P.queue + ((base + lane) & P.qmask) * 2;
+                u64 a, b;
+                for (;;) {  // the producer reserved this slot and is writing it
+                  a = gld(slot);
+                  b = gld(slot + 1);
+                  if (a != 0 && b != 0) break;
+                  if (wall_clock64() > t_end) {
+                    fail(L, ERR_TIMEOUT);
+                    a = b = P_ERA;  // skippable pair: dropped without effect
+                    break;
+                  }
+                  __builtin_amdgcn_s_sleep(1);
+                }
+                gst(slot, 0);
+                gst(slot + 1, 0);
+                // the rings are empty, so k <= 64 entries always fit
+                const u32 c = redex_class(a, b);
+                const u32 idx = HVM_LDS_FAA(&top_c[c], 1u);
+                const u32 i = (ring_base(c) + (idx & (ring_cap(c) - 1))) * 2;
+                bag[i] = a;
+                bag[i + 1] = b;
+              }
+              __builtin_amdgcn_wave_barrier();
+              continue;
+            }
+            if (use_mbox) {  // nothing claimed: back into the idle set
+              if (lane == 0)
+                __hip_atomic_fetch_or(&P.idle_bits[gwave >> 6], 1ull << (gwave & 63), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+              in_set = true;
+            }
+          }
+        }
+      }
+      if (use_mbox) {
+        wave_sleep(MBOX_POLL);
+      } else {
         wave_sleep(backoff);
         backoff = backoff < MAX_BACKOFF ? backoff * 2 + (gwave & 3) : MAX_BACKOFF - (gwave & 63);
-        continue;
       }
-      backoff = 1 + (gwave & 7);
-      active = true;
-      st[ST_CLAIMS]++;
-      st[ST_CLAIMED] += k;
-      u64 base = 0;
-      if (lane == 0) base = __hip_atomic_fetch_add(&g->q_head, (u64)k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      base = __shfl(base, 0, 64);
-      if (lane < k) {
-        u64 *slot = P.queue + ((base + lane) & P.qmask) * 2;
-        u64 a, b;
-        for (;;) {  // the producer reserved this slot and is writing it
-          a = gld(slot);
-          b = gld(slot + 1);
-          if (a != 0 && b != 0) break;
-          if (wall_clock64() > t_end) {
-            fail(L, ERR_TIMEOUT);
-            a = b = P_ERA;  // skippable pair: dropped without effect
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-        }
-        gst(slot, 0);
-        gst(slot + 1, 0);
-        // the rings are empty, so k <= 64 entries always fit
-        u32 c = redex_class(a, b);
-        u32 idx = HVM_LDS_FAA(&top_c[c], 1u);
-        u32 i = (ring_base(c) + (idx & (ring_cap(c) - 1))) * 2;
-        bag[i] = a;
-        bag[i + 1] = b;
-      }
-      __builtin_amdgcn_wave_barrier();
       continue;
     }
 
@@ -462,6 +535,50 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     u32 spill = 0;
     if (snc > ring_cap(sc) * 3 / 4) spill = snc - ring_cap(sc) / 2;  // keep room for one iteration's pushes
     else if (n > 64 + 16 && idle_waves > 0 && queued < idle_waves * 64) spill = snc < n - 64 ? snc : n - 64;
+    // with idle waves waiting on their mailboxes, every redex beyond one
+    // iteration's worth goes to one of them
+    if (use_mbox && idle_waves > 0 && n > MBOX_KEEP && spill == 0) spill = snc < n - MBOX_KEEP ? snc : n - MBOX_KEEP;
+    if (spill && use_mbox && idle_waves > 0) {
+      // hand up to MBOX_PAIRS of the excess to one idle wave
+      const u32 give = spill < MBOX_PAIRS ? spill : MBOX_PAIRS;
+      const u32 start = (gwave * 7u) % P.idle_words;
+      u32 wi = start + lane;
+      wi = wi >= P.idle_words ? wi - P.idle_words : wi;
+      const u64 w = lane < P.idle_words ? gld(&P.idle_bits[wi]) : 0;
+      const u64 ball = __ballot(w != 0);
+      if (ball) {
+        const u32 first = (u32)__builtin_ctzll(ball);
+        const u32 word = uniform(__shfl(wi, first, 64));
+        const u64 bits = __shfl(w, first, 64);
+        const u32 r = gwave & 63;
+        const u64 rot = (bits >> r) | (r ? bits << (64 - r) : 0);
+        const u32 bit = ((u32)__builtin_ctzll(rot) + r) & 63;
+        u64 old = 0;
+        if (lane == 0)
+          old = __hip_atomic_fetch_and(&P.idle_bits[word], ~(1ull << bit), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const bool got = ((__shfl(old, 0, 64) >> bit) & 1) != 0;
+        const u32 target = word * 64 + bit;
+        if (got && target < P.total_waves) {
+          if (lane == 0) __hip_atomic_fetch_add(&g->S, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          publish();  // the nodes these redexes reach, before the mailbox entries
+          u64 *box = P.mbox + (u64)target * MBOX_WORDS;
+          const u32 bot0 = uniform(__shfl(my_bot, sc, 64));
+          if (lane < give) {
+            const u32 s2 = (ring_base(sc) + ((bot0 + lane) & (ring_cap(sc) - 1))) * 2;
+            gst(box + 16 + 2 * lane, bag[s2]);
+            gst(box + 16 + 2 * lane + 1, bag[s2 + 1]);
+          }
+          publish();
+          if (lane == 0) {
+            gst(box, (u64)give);
+            bot_c[sc] = bot0 + give;
+          }
+          st[ST_SPILLS]++;
+          st[ST_SPILLED] += give;
+          spill = 0;
+        }
+      }
+    }
     if (spill) {
       const u64 t_sp = wall_clock64();
       st[ST_SPILLS]++;
