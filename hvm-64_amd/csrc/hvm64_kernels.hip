@@ -420,8 +420,11 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
       continue;
     }
 
-    // ---- choose the fullest class (lowest index on ties)
-    u32 key = lane < NCLS ? (my_n << 4) | (15 - lane) : 0;
+    // ---- choose a ring: the fullest of the anni / comm / other rings if any
+    // holds redexes (they are cheap and release the work behind them, and
+    // definition calls meanwhile gather into fuller iterations), else the
+    // fullest definition-call ring; lowest index on ties
+    u32 key = lane < NCLS ? ((lane < CLS_CALL && my_n > 0) ? (1u << 24) : 0u) | (my_n << 4) | (15 - lane) : 0;
 #pragma unroll
     for (u32 o = 1; o < 16; o <<= 1) {
       u32 other = __shfl_xor(key, o, 64);
@@ -429,7 +432,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     }
     key = uniform(key);
     const u32 c = 15 - (key & 15);
-    const u32 nc = key >> 4;
+    const u32 nc = (key >> 4) & 0xFFFFF;
     const u32 top = uniform(__shfl(my_top, c, 64));
     const u32 bot = uniform(__shfl(my_bot, c, 64));
 
