@@ -172,6 +172,9 @@ NOINL AllocCursor alloc_slow(u64 *heap, Globals *g, u64 wave_nodes, u64 ov_node0
     c.slice_rest--;
   }
   c.ov_allocs++;
+  // Overflow slots are shared: a slot is taken by swapping its first word
+  // from FREE (so two lanes whose runs alias the same slot a lap apart can
+  // never both take it), and kept only if its second word is FREE too.
   for (u64 miss = 0; miss <= ov_nodes; miss++) {
     if (c.ov_left == 0) {
       c.ov_cur = HVM_FAA(&g->ov_pos, (u64)OV_RUN);
@@ -180,14 +183,13 @@ NOINL AllocCursor alloc_slow(u64 *heap, Globals *g, u64 wave_nodes, u64 ov_node0
     u64 pos = c.ov_cur++;
     c.ov_left--;
     u64 node = ov_node0 + (pos & (ov_nodes - 1));
-    if (pos < ov_nodes) {
-      c.addr = HVM_HEAP_OFF + node * 16;
-      return c;
-    }
-    u64 w0 = HVM_LD(heap + 2 * node), w1 = HVM_LD(heap + 2 * node + 1);
-    if (w0 == P_FREE && w1 == P_FREE) {
-      c.addr = HVM_HEAP_OFF + node * 16;
-      return c;
+    u64 expect = P_FREE;
+    if (HVM_CAS(heap + 2 * node, &expect, P_NONE)) {
+      if (HVM_LD(heap + 2 * node + 1) == P_FREE) {
+        c.addr = HVM_HEAP_OFF + node * 16;
+        return c;
+      }
+      HVM_ST(heap + 2 * node, P_FREE);  // second word in use: not a free node
     }
   }
   c.err = ERR_OOM;

@@ -234,3 +234,23 @@ def test_batch_mix_against_oracle(gpu_heap, oracle):
         for k in total:
             total[k] += ref[v]["rwts"][k]
     assert batch.rwts.as_dict() == total
+
+
+@pytest.mark.parametrize("k", [13, 14])
+def test_small_heap_reuses_memory(oracle, k):
+    """A heap far smaller than the allocations of the reduction: every lane's
+    allocation slice wraps several times and the shared overflow ring is
+    used, so freed nodes are reused while the net is live."""
+    import hvm64
+    from hvm64 import configs
+    print("case small heap bitonic", k, flush=True)
+    heap = hvm64.Heap(1 << 30)
+    try:
+        heap.set_timeout(120)
+        text = configs.bitonic(k, 11)
+        rw, nf, _ = hvm64.normal_with(text, heap=heap)
+        assert nf == configs.bitonic_expected(k, 11)
+        ref = oracle.run(text, heap_words=1 << 28)
+        assert nf == ref["nf"] and rw.as_dict() == ref["rwts"]
+    finally:
+        heap.close()
