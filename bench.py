@@ -45,10 +45,11 @@ def workload(name, size, seed):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(name, size, seed, seconds_budget=20.0):
+def cpu_baseline(name, size, seed):
     """Times the oracle's parallel_normal (runtime/src/parallel.rs restated)
-    with T = 2^floor(log2(cores)) threads, cores capped at 16, on a bounded
-    sample: the same workload at a size that takes a few seconds."""
+    with T = 2^floor(log2(cores)) threads, cores capped at 16, and its
+    single-threaded normal(), on the same workload at the same size (bitonic
+    2^16: about 360M rewrites, 10-30 s of CPU work for both runs)."""
     try:
         import oracle_lib
         from hvm64 import configs
@@ -59,7 +60,7 @@ def cpu_baseline(name, size, seed, seconds_budget=20.0):
     threads = 1
     while threads * 2 <= cores:
         threads *= 2
-    small = {"bitonic": max(size - 3, 6), "tree_sum": max(size - 1, 6), "church": max(size - 2, 4)}[name]
+    small = size
     text = workload(name, small, seed)[1]
     single = o.run(text, heap_words=1 << 28, threads=0)
     multi = o.run(text, heap_words=1 << 28, threads=threads)

@@ -164,8 +164,13 @@ void validate_book(const hvm_gpu *h) {
   }
 }
 
+// Call ring group of a definition by instruction count: the lanes of a call
+// iteration wait for the longest definition among them.
+u32 call_group(u32 ilen) { return ilen <= 4 ? 0 : ilen <= 8 ? 1 : ilen <= 16 ? 2 : 3; }
+
 void upload_book(hvm_gpu *h) {
   validate_book(h);
+  for (auto &d : h->defs_host) d.group = call_group(d.ilen);
   dfree(h->d_defs);
   dfree(h->d_instrs);
   dfree(h->d_labbits);
@@ -600,6 +605,20 @@ int hvm_gpu_shortcuts(hvm_gpu *h, uint64_t out[3]) {
   });
 }
 
+int hvm_gpu_snapshot(hvm_gpu *h, hvm_net_image *img, uint64_t first, uint64_t n, uint64_t *dst) {
+  return guarded(h, [&] {
+    if (!img) throw Error(HVM_E_ARG, "null image");
+    if (first > h->heap_words || n > h->heap_words - first) throw Error(HVM_E_ARG, "word range outside the heap");
+    img->root_cell = h->root_cell;
+    img->heap_base = HVM_HEAP_OFF;
+    img->heap_words = h->heap_words;
+    if (dst && n) {
+      hip_check(hipStreamSynchronize(h->stream), "snapshot");
+      hip_check(hipMemcpy(dst, h->heap + first, n * sizeof(u64), hipMemcpyDeviceToHost), "copy heap image");
+    }
+  });
+}
+
 int hvm_gpu_readback(hvm_gpu *h, char *buf, uint64_t cap, uint64_t *len) {
   int rc = 0;
   int g = guarded(h, [&] {
@@ -697,7 +716,7 @@ int hvm_gpu_sched_stats(hvm_gpu *h, uint64_t out[20]) {
     out[9] = h->last.rw[RW_ALLOC_MISS];
     out[10] = h->last.rw[RW_OV_ALLOC];
     for (u32 k = 0; k < 8; k++) out[11 + k] = h->last.st[ST_CLS_IT + k];
-    out[19] = 0;
+    out[19] = h->last.rw[RW_POPS];
   });
 }
 

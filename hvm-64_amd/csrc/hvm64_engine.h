@@ -25,7 +25,8 @@ struct DefRec {
   u32 labs_off;   // first word of the label bitset in the label array
   u32 labs_words; // bitset length in words
   u32 min_safe;   // LabSet::min_safe (also the label of Ref ports to this def)
-  u32 pad[2];
+  u32 group;      // call ring of this definition's calls (by instruction count)
+  u32 pad;
 };
 
 // Rewrite counter slots (runtime.rs:72-79) and interpreter shortcut counters

@@ -308,7 +308,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
         if (k > 0) {  // a delivery: the sender already counted this wave active
           if (lane < k) {
             const u64 a = gld(my_box + 16 + 2 * lane), b = gld(my_box + 16 + 2 * lane + 1);
-            const u32 c = redex_class(a, b);  // the rings are empty: k <= 64 entries fit
+            const u32 c = redex_class(a, b, L.book, P.ndefs);  // the rings are empty: k <= 64 entries fit
             const u32 idx = HVM_LDS_FAA(&top_c[c], 1u);
             const u32 i = (ring_base(c) + (idx & (ring_cap(c) - 1))) * 2;
             bag[i] = a;
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
                 gst(slot, 0);
                 gst(slot + 1, 0);
                 // the rings are empty, so k <= 64 entries always fit
-                const u32 c = redex_class(a, b);
+                const u32 c = redex_class(a, b, L.book, P.ndefs);
                 const u32 idx = HVM_LDS_FAA(&top_c[c], 1u);
                 const u32 i = (ring_base(c) + (idx & (ring_cap(c) - 1))) * 2;
                 bag[i] = a;

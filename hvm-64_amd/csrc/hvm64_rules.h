@@ -19,6 +19,11 @@ typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 #define HVM_TO_LDS(p) ((lds_u64 *)(p))
 // Retire every outstanding vector memory operation of this wave (vmcnt(0)).
 #define HVM_PUBLISH() __builtin_amdgcn_s_waitcnt(0x0F70)
+// The same wait, placed where nothing is outstanding but loads whose results
+// are dead on some paths: it clears the compiler's record of them, which
+// would otherwise put a vmcnt wait (that also waits for the stores issued
+// since) at the top of every interpreter step.
+#define HVM_SETTLE_LOADS() __builtin_amdgcn_s_waitcnt(0x0F70)
 #else
 #define DEV static inline
 #define NOINL static
@@ -31,6 +36,7 @@ typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 typedef unsigned long long lds_u64;
 #define HVM_TO_LDS(p) ((lds_u64 *)(p))
 #define HVM_PUBLISH() ((void)0)
+#define HVM_SETTLE_LOADS() ((void)0)
 #endif
 
 namespace hvm {
@@ -288,14 +294,20 @@ DEV void global_push1(Lane &L, u64 a, u64 b) {
 }
 
 // The rule a redex will take, grouped: the arms of interact() that run
-// anni2, comm22, a definition call (a non-ERA Ref; one ring per definition
-// modulo NCALL), and everything else.
-DEV u32 redex_class(u64 a, u64 b) {
+// anni2, comm22, a definition call (a non-ERA Ref), and everything else.
+// Calls go to the ring of the definition's group (DefRec::group, by
+// instruction count), read from the book in LDS; a wave runs one call ring
+// per iteration and waits for its longest definition, so definitions of
+// similar length share a ring.  Without the LDS book: definition modulo NCALL.
+DEV u32 redex_class(u64 a, u64 b, const lds_u64 *book, u32 ndefs) {
   u32 at = ptag(a), bt = ptag(b);
   const bool aref = at == T_REF && a != P_ERA, bref = bt == T_REF && b != P_ERA;
   if (aref || bref) {
     const u64 addr = paddr(aref ? a : b);
-    return CLS_CALL + (addr_in_heap(addr) ? 0u : (u32)(addr >> 5) % NCALL);
+    if (addr_in_heap(addr)) return CLS_CALL;
+    const u32 di = def_index(addr);
+    const u32 g = book ? (di < ndefs ? (u32)book[4 * di + 3] : 0u) : di;
+    return CLS_CALL + g % NCALL;
   }
   if (at == bt) {
     if (at == T_OP || at == T_SWI) return CLS_ANNI;
@@ -314,7 +326,7 @@ DEV void redux(Lane &L, u64 a, u64 b) {
     return;
   }
   if (L.bag) {
-    u32 c = redex_class(a, b);
+    u32 c = redex_class(a, b, L.book, L.P->ndefs);
     u32 idx = HVM_LDS_FAA(&L.bag_top[c], 1u);
     if (idx - L.bag_bot[c] < ring_cap(c)) {
       u32 k = (ring_base(c) + (idx & (ring_cap(c) - 1))) * 2;
@@ -505,7 +517,8 @@ DEV DefRec load_def(const Lane &L, u32 di) {
     d.labs_off = (u32)(w1 >> 32);
     d.labs_words = (u32)w2;
     d.min_safe = (u32)(w2 >> 32);
-    d.pad[0] = d.pad[1] = 0;
+    d.group = (u32)L.book[4 * di + 3];
+    d.pad = 0;
     return d;
   }
   return L.P->defs[di];
@@ -732,6 +745,7 @@ DEV void interp_begin(Lane &L, Interp &I, u32 di, u64 t0) {
 // (every instruction appends at most one link).  Instructions are fetched
 // two at a time (32 bytes); exec_step has one call site (code size).
 DEV void interp_run(Lane &L, Interp &I, u32 &np) {
+  HVM_SETTLE_LOADS();
   while (I.left > 0 && np < LINK_CAP) {
     u32 n = I.left < 2 ? I.left : 2;
     if (n > LINK_CAP - np) n = LINK_CAP - np;
@@ -750,6 +764,10 @@ DEV void interp_run(Lane &L, Interp &I, u32 &np) {
         w2 = I.ip[2];
         w3 = I.ip[3];
       }
+      // Wait for these loads here: otherwise the join below waits on every
+      // outstanding vector memory operation (stores included) on the LDS
+      // path too.
+      HVM_SETTLE_LOADS();
     }
 #pragma clang loop unroll(disable)
     for (u32 j = 0; j < n; j++) exec_step(L, I, j ? w2 : w0, j ? w3 : w1, np);

@@ -33,10 +33,16 @@ EXPORTS = [
     "hvm_gpu_reduce", "hvm_gpu_rewrites", "hvm_gpu_shortcuts", "hvm_gpu_readback", "hvm_gpu_last_kernel_ms",
     "hvm_gpu_grid", "hvm_gpu_set_timeout", "hvm_gpu_sched_stats", "hvm_gpu_open_grid",
     "hvm_gpu_boot_batch", "hvm_gpu_normal_batch", "hvm_gpu_root_ports", "hvm_gpu_readback_root",
+    "hvm_gpu_snapshot",
     "hvm_book_print", "hvm_book_label_sets", "hvm_net_apply_tree",
 ]
 
 _lib = None
+
+
+class NetImage(ctypes.Structure):
+    """hvm_net_image: root var cell, port address of heap word 0, heap words."""
+    _fields_ = [("root_cell", ctypes.c_uint64), ("heap_base", ctypes.c_uint64), ("heap_words", ctypes.c_uint64)]
 
 
 class HvmError(RuntimeError):
@@ -82,6 +88,7 @@ def load_library(path=None):
     lib.hvm_gpu_normal_batch.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(u64)]
     lib.hvm_gpu_root_ports.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(u64), ctypes.POINTER(u64)]
     lib.hvm_gpu_readback_root.argtypes = [P, u64, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
+    lib.hvm_gpu_snapshot.argtypes = [P, ctypes.POINTER(NetImage), u64, u64, ctypes.POINTER(u64)]
     for name in ("hvm_book_print", "hvm_book_label_sets"):
         getattr(lib, name).argtypes = [ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
     lib.hvm_net_apply_tree.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
@@ -268,8 +275,16 @@ class Net:
         keys = ["busy_iters", "idle_iters", "busy_ticks", "spills", "spilled", "claims", "claimed",
                 "interact_ticks", "spill_ticks", "alloc_miss", "ov_alloc",
                 "anni_iters", "comm_iters", "call_iters", "other_iters",
-                "anni_ticks", "comm_ticks", "call_ticks", "other_ticks"]
+                "anni_ticks", "comm_ticks", "call_ticks", "other_ticks", "pops"]
         return dict(zip(keys, [int(x) for x in out]))
+
+    def snapshot(self, first=0, n=0):
+        """(NetImage, words): the raw device image, with heap words
+        [first, first + n) copied to the host as a list of ints."""
+        img = NetImage()
+        buf = (ctypes.c_uint64 * n)() if n else None
+        self.heap._check(self.heap._lib.hvm_gpu_snapshot(self.heap._h, ctypes.byref(img), first, n, buf))
+        return img, (list(buf) if n else [])
 
     def last_kernel_ms(self):
         ms = ctypes.c_double()

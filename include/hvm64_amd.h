@@ -147,6 +147,20 @@ int hvm_gpu_shortcuts(hvm_gpu *h, uint64_t out[3]);
  * returns HVM_E_TRUNC with the needed length (without NUL) in *len. */
 int hvm_gpu_readback(hvm_gpu *h, char *buf, uint64_t cap, uint64_t *len);
 
+/* Raw image of the device net, for host-side tools that read the net in
+ * place as Host::readback does (host/src/readback.rs:23-87): the root's var
+ * cell (a port naming it is where readback starts), the port address of
+ * heap word 0 and the heap size in words.  When `dst` is non-null, heap
+ * words [first, first + n) are copied into it.  A heap word holds one port;
+ * a binary node at port address A is the two words at A and A + 8; a var
+ * cell holds the port its wire leads to once the far side has linked. */
+typedef struct {
+  uint64_t root_cell;
+  uint64_t heap_base;
+  uint64_t heap_words;
+} hvm_net_image;
+int hvm_gpu_snapshot(hvm_gpu *h, hvm_net_image *img, uint64_t first, uint64_t n, uint64_t *dst);
+
 /* Duration in milliseconds of the last reduction kernel (HIP events on the
  * kernel's stream), and the reduce grid: blocks and waves. */
 int hvm_gpu_last_kernel_ms(hvm_gpu *h, double *ms);
@@ -163,7 +177,8 @@ int hvm_gpu_grid(hvm_gpu *h, uint32_t *blocks, uint32_t *waves);
  * interaction rules, [8] ticks spilling; since the net was reset: [9] ring
  * slots found in use on a later lap, [10] overflow-ring allocations;
  * [11..14] iterations that ran anni / comm / call / other redexes,
- * [15..18] ticks inside the rules for each of those classes; [19] 0. */
+ * [15..18] ticks inside the rules for each of those classes; [19] redexes
+ * popped since the net was reset. */
 int hvm_gpu_sched_stats(hvm_gpu *h, uint64_t out[20]);
 
 /* Host-only book utilities (no device).  Text results follow

@@ -254,3 +254,49 @@ def test_small_heap_reuses_memory(oracle, k):
         assert nf == ref["nf"] and rw.as_dict() == ref["rwts"]
     finally:
         heap.close()
+
+
+def _image_text(img, read, port):
+    """Prints the tree a port leads to in the raw heap image, following var
+    cells as host/src/readback.rs:38-87 does (lab-0 Ctr, ERA and u24 only)."""
+    NONE, MASK = 0xFFFF_FFFF_FFFF_FFF0, 0x0000_FFFF_FFFF_FFF8
+    while port & 7 in (0, 1):  # a wire endpoint: the cell holds its target once linked
+        v = read(port & MASK)
+        assert v != NONE and v != 1 << 63, "unlinked or freed wire in a closed normal form"
+        port = v
+    tag = port & 7
+    if port == 2:
+        return "*"
+    if tag == 3:
+        raw = port & 0xFFFF_FFFF
+        assert (raw >> 3) & 31 == 0, "only u24 numbers are printed here"
+        return str(raw >> 8)
+    assert tag == 7 and port >> 48 == 0, f"unexpected port {port:#x}"
+    a = port & MASK
+    return f"({_image_text(img, read, read(a))} {_image_text(img, read, read(a + 8))})"
+
+
+@pytest.mark.parametrize("text,expected", [
+    ("@main = (1 (2 (* 3)))", "(1 (2 (* 3)))"),
+    ("@main = a\n& @f ~ (1 a)\n@f = (x (x 2))", "(1 2)"),
+    ("@main = a\n& @f ~ (4 (5 a))\n@f = (x (y (y (x *))))", "(5 (4 *))"),
+])
+def test_snapshot_image_matches_readback(gpu_heap, text, expected):
+    """hvm_gpu_snapshot: walking the raw heap image on the host from the root
+    cell gives the same tree as the device readback."""
+    print("case snapshot", repr(text), flush=True)
+    nf, _, net = run_gpu(gpu_heap, text)
+    assert nf == expected
+    img, _ = net.snapshot()
+    assert img.heap_base == 1 << 32 and img.heap_words > 0
+    cache = {}
+
+    def read(addr):
+        w = (addr - img.heap_base) >> 3
+        assert 0 <= w < img.heap_words
+        if w not in cache:
+            _, words = net.snapshot(w, 1)
+            cache[w] = words[0]
+        return cache[w]
+
+    assert _image_text(img, read, img.root_cell | 1) == expected
