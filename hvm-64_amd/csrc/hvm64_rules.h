@@ -82,6 +82,22 @@ DEV u64 *hw(const Params &P, u64 addr) {
   return P.heap + (off >> 3);
 }
 DEV u64 ld(const Params &P, u64 addr) { return HVM_LD(hw(P, addr)); }
+// Both words of a node (16-byte aligned) in one agent-scope store.  A store
+// ahead of a load delays that load until the store is acknowledged (the
+// vector memory counter retires in order), so a node costs one store, not
+// two.  The compiler does not count this store in its waits; since the
+// counter retires in order, an uncounted store only makes later waits
+// longer, never shorter, and a full wait (publish) covers it.
+DEV void st16(u64 *p, u64 w0, u64 w1) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef u64 u64x2 __attribute__((ext_vector_type(2)));
+  const u64x2 d = {w0, w1};
+  asm volatile("global_store_dwordx4 %0, %1, off sc1" ::"v"(p), "v"(d) : "memory");
+#else
+  HVM_ST(p, w0);
+  HVM_ST(p + 1, w1);
+#endif
+}
 DEV void st(const Params &P, u64 addr, u64 v) { HVM_ST(hw(P, addr), v); }
 DEV u64 xchg(const Params &P, u64 addr, u64 v) { return HVM_XCHG(hw(P, addr), v); }
 DEV u64 gld(u64 *p) { return HVM_LD(p); }
@@ -227,11 +243,7 @@ DEV u64 alloc_node(Lane &L) {
 
 // Two node words.  Agent-scope stores: a plain store may stay in the
 // writing XCD's L2, where lanes on the other XCDs would not see it.
-DEV void st2(const Params &P, u64 addr, u64 w0, u64 w1) {
-  u64 *p = hw(P, addr);
-  HVM_ST(p, w0);
-  HVM_ST(p + 1, w1);
-}
+DEV void st2(const Params &P, u64 addr, u64 w0, u64 w1) { st16(hw(P, addr), w0, w1); }
 
 // A fresh pair of var cells (both words of one node): cells n and n + 8.
 DEV u64 alloc_vars2(Lane &L) {
@@ -261,8 +273,7 @@ DEV void take(Lane &L, u64 p, u64 &x, u64 &y) {
   u64 a = paddr(p);
   x = ld(*L.P, a);
   y = ld(*L.P, a + 8);
-  st(*L.P, a, P_FREE);
-  st(*L.P, a + 8, P_FREE);
+  st16(hw(*L.P, a & ~15ull), P_FREE, P_FREE);
 }
 
 // Retire this lane's stores before a port to new memory becomes visible.
@@ -566,9 +577,7 @@ DEV void t_set(Interp &I, u32 s, u64 v) {
 
 // Two words of a node this lane has just allocated (always in the heap).
 DEV void st2_new(const Params &P, u64 addr, u64 w0, u64 w1) {
-  u64 *p = P.heap + ((addr - HVM_HEAP_OFF) >> 3);
-  HVM_ST(p, w0);
-  HVM_ST(p + 1, w1);
+  st16(P.heap + ((addr - HVM_HEAP_OFF) >> 3), w0, w1);
 }
 
 // Two fresh nodes: consecutive slice positions on the first lap.
