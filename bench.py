@@ -257,12 +257,18 @@ def main():
                     help="batch: nets timed with one launch sequence each (the per-net baseline)")
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--heap-gb", type=float, default=32.0)
+    ap.add_argument("--heap-gb", type=float, default=None,
+                    help="heap size (default 192 GiB of the MI355X's 288 GB; serve: split over the slots)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before the steps, copy a buffer of this many MiB on the device (a known-size "
                          "kernel that calibrates HBM byte counters in a profile)")
     args = ap.parse_args()
+    if args.heap_gb is None:
+        # a larger heap keeps each lane's allocation slice on its first lap
+        # (no reads to find free slots): bitonic 2^16 runs at ~1640 MIPS with
+        # 32 GiB and ~1890 with 192 GiB
+        args.heap_gb = 192.0 / args.slots if args.workload == "serve" else 192.0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs one net per "
               f"device and this build has no multi-GPU path).", file=sys.stderr)
