@@ -208,8 +208,13 @@ void reset_net(hvm_gpu *h, bool fill) {
 }
 
 // Run the persistent reduction kernel with a redex budget.
-void run_reduce(hvm_gpu *h, u64 limit) {
+// `fill`: for batched nets, where nearly every wave is busy and the global
+// ring holds the excess, a wave whose rings run low claims from the global
+// ring without going idle first (fuller iterations; a single net, with many
+// idle waves, gains nothing from it).
+void run_reduce(hvm_gpu *h, u64 limit, bool fill = false) {
   Params P = make_params(h, limit);
+  P.fill = fill ? 1 : 0;
   u64 zero = 0;
   u64 zst[ST_SLOTS] = {};
   hip_check(hipMemcpyAsync(&h->g->done, &zero, sizeof(u64), hipMemcpyHostToDevice, h->stream), "budget");
@@ -542,7 +547,7 @@ int hvm_gpu_normal_batch(hvm_gpu *h, uint32_t n, uint64_t *roots) {
       hip_check(hipMemcpyAsync(roots, d_roots, (size_t)n * 8, hipMemcpyDeviceToHost, h->stream), "download roots");
       sync_globals(h);
     }
-    run_reduce(h, ~0ull);
+    run_reduce(h, ~0ull, true);
   });
 }
 

@@ -108,7 +108,7 @@ struct Params {
   u64 *idle_bits;      // one bit per wave of the reduce grid: the wave is idle
   u64 *mbox;           // per-wave mailboxes (MBOX_WORDS each)
   u32 idle_words;      // words in idle_bits
-  u32 _pad2;
+  u32 fill;            // busy waves short of work also claim from the global ring
   u64 limit;           // redex budget for reduce(limit); ~0 = unlimited
   u64 timeout_ticks;   // wall-clock (100 MHz) budget of a launch; 0 = none
 };

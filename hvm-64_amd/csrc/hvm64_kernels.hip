@@ -171,6 +171,7 @@ static const u32 MAX_BACKOFF = 16384;  // idle backoff cap, in s_sleep(2) units 
 static const u32 MBOX_POLL = 16;       // s_sleep(2) units between mailbox polls (~1 us)
 static const u32 IDLE_CHECK = 64;      // mailbox polls between reads of S (a power of two)
 static const u32 MBOX_KEEP = 80;       // redexes a busy wave keeps before handing some off
+static const u32 FILL_BELOW = 32;      // batched nets: pending redexes under which a busy wave claims
 
 DEV void wave_sleep(u32 units) {
   for (u32 z = 0; z < units; z++) __builtin_amdgcn_s_sleep(2);
@@ -180,6 +181,36 @@ DEV u32 uniform(u32 v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Moves `cnt` redexes, the oldest of class `c`, from the wave's rings to the
 // global ring (all lanes; lane 0 updates the counters).  Returns S after.
+// Takes the next k (<= 64) redexes of the global ring, whose count in S this
+// wave has already claimed, into its rings (the caller checks they fit).
+DEV void claim_global(const Params &P, Lane &L, u64 *bag, u32 *top_c, u32 k, u32 lane, u64 t_end) {
+  u64 base = 0;
+  if (lane == 0) base = __hip_atomic_fetch_add(&P.g->q_head, (u64)k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  base = __shfl(base, 0, 64);
+  if (lane < k) {
+    u64 *slot = P.queue + ((base + lane) & P.qmask) * 2;
+    u64 a, b;
+    for (;;) {  // the producer reserved this slot and is writing it
+      a = gld(slot);
+      b = gld(slot + 1);
+      if (a != 0 && b != 0) break;
+      if (wall_clock64() > t_end) {
+        fail(L, ERR_TIMEOUT);
+        a = b = P_ERA;  // skippable pair: dropped without effect
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    gst(slot, 0);
+    gst(slot + 1, 0);
+    const u32 c = redex_class(a, b, L.book, P.ndefs);
+    const u32 idx = HVM_LDS_FAA(&top_c[c], 1u);
+    const u32 i = (ring_base(c) + (idx & (ring_cap(c) - 1))) * 2;
+    bag[i] = a;
+    bag[i + 1] = b;
+  }
+}
+
 DEV u64 spill_to_global(const Params &P, u64 *bag, u32 *bot_c, u32 c, u32 cnt, u32 lane, u32 bot, u32 &err) {
   // the nodes these redexes reach must be in memory before a claimer can
   // read the ring slots (which it may do before S counts them)
@@ -373,32 +404,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
               active = true;
               st[ST_CLAIMS]++;
               st[ST_CLAIMED] += k;
-              u64 base = 0;
-              if (lane == 0) base = __hip_atomic_fetch_add(&g->q_head, (u64)k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              base = __shfl(base, 0, 64);
-              if (lane < k) {
-                u64 *slot = P.queue + ((base + lane) & P.qmask) * 2;
-                u64 a, b;
-                for (;;) {  // the producer reserved this slot and is writing it
-                  a = gld(slot);
-                  b = gld(slot + 1);
-                  if (a != 0 && b != 0) break;
-                  if (wall_clock64() > t_end) {
-                    fail(L, ERR_TIMEOUT);
-                    a = b = P_ERA;  // skippable pair: dropped without effect
-                    break;
-                  }
-                  __builtin_amdgcn_s_sleep(1);
-                }
-                gst(slot, 0);
-                gst(slot + 1, 0);
-                // the rings are empty, so k <= 64 entries always fit
-                const u32 c = redex_class(a, b, L.book, P.ndefs);
-                const u32 idx = HVM_LDS_FAA(&top_c[c], 1u);
-                const u32 i = (ring_base(c) + (idx & (ring_cap(c) - 1))) * 2;
-                bag[i] = a;
-                bag[i + 1] = b;
-              }
+              claim_global(P, L, bag, top_c, k, lane, t_end);  // the rings are empty: k <= 64 fit
               __builtin_amdgcn_wave_barrier();
               continue;
             }
@@ -541,6 +547,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     // with idle waves waiting on their mailboxes, every redex beyond one
     // iteration's worth goes to one of them
     if (use_mbox && idle_waves > 0 && n > MBOX_KEEP && spill == 0) spill = snc < n - MBOX_KEEP ? snc : n - MBOX_KEEP;
+    const u32 spill0 = spill;
     if (spill && use_mbox && idle_waves > 0) {
       // hand up to MBOX_PAIRS of the excess to one idle wave
       const u32 give = spill < MBOX_PAIRS ? spill : MBOX_PAIRS;
@@ -590,6 +597,38 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
       S_cache = spill_to_global(P, bag, &bot_c[sc], sc, spill, lane, uniform(__shfl(my_bot, sc, 64)), qerr);
       if (qerr) fail(L, qerr);
       st[ST_SPILL_TICKS] += wall_clock64() - t_sp;
+    }
+    // ---- batched nets: a busy wave short of work also claims from the
+    // global ring, while few waves are idle.  With n < 64 pending, every
+    // ring has room for the 64 - n it may take.
+    if (P.fill && !limited && spill0 == 0 && n < FILL_BELOW && (S_cache >> 32) > 0 &&
+        idle_waves * 8 < P.total_waves) {
+      u32 k = 0;
+      u64 s_new = S_cache;
+      if (lane == 0) {
+        u64 s = S_cache;
+        for (int tries = 0; tries < 2; tries++) {
+          const u64 q = s >> 32;
+          if (q == 0) break;
+          const u64 room = 64 - n;
+          const u64 want = q < room ? q : room;
+          const u64 ns = s - (want << 32);
+          if (__hip_atomic_compare_exchange_strong(&g->S, &s, ns, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT)) {
+            k = (u32)want;
+            s_new = ns;
+            break;
+          }
+          s_new = s;
+        }
+      }
+      k = uniform(__shfl(k, 0, 64));
+      S_cache = __shfl(s_new, 0, 64);
+      if (k > 0) {
+        st[ST_CLAIMS]++;
+        st[ST_CLAIMED] += k;
+        claim_global(P, L, bag, top_c, k, lane, t_end);
+      }
     }
     __builtin_amdgcn_wave_barrier();
     st[ST_BUSY_TICKS] += wall_clock64() - t_it;
