@@ -245,6 +245,41 @@ DEV u64 alloc_node(Lane &L) {
 // writing XCD's L2, where lanes on the other XCDs would not see it.
 DEV void st2(const Params &P, u64 addr, u64 w0, u64 w1) { st16(hw(P, addr), w0, w1); }
 
+// Both words of a published node (immutable until taken) in one 16-byte
+// agent-scope load; two nodes: two loads and one wait.  The compiler does not
+// track these loads, so the asm waits for them itself.
+DEV void ld16x2(const u64 *p, const u64 *q, u64 &p0, u64 &p1, u64 &q0, u64 &q1) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef u64 u64x2 __attribute__((ext_vector_type(2)));
+  u64x2 dp, dq;
+  asm volatile("global_load_dwordx4 %0, %2, off sc1\n\tglobal_load_dwordx4 %1, %3, off sc1\n\ts_waitcnt vmcnt(0)"
+               : "=&v"(dp), "=&v"(dq)
+               : "v"(p), "v"(q)
+               : "memory");
+  p0 = dp.x;
+  p1 = dp.y;
+  q0 = dq.x;
+  q1 = dq.y;
+#else
+  p0 = HVM_LD(p);
+  p1 = HVM_LD(p + 1);
+  q0 = HVM_LD(q);
+  q1 = HVM_LD(q + 1);
+#endif
+}
+DEV void ld16(const u64 *p, u64 &p0, u64 &p1) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef u64 u64x2 __attribute__((ext_vector_type(2)));
+  u64x2 dp;
+  asm volatile("global_load_dwordx4 %0, %1, off sc1\n\ts_waitcnt vmcnt(0)" : "=v"(dp) : "v"(p) : "memory");
+  p0 = dp.x;
+  p1 = dp.y;
+#else
+  p0 = HVM_LD(p);
+  p1 = HVM_LD(p + 1);
+#endif
+}
+
 // A fresh pair of var cells (both words of one node): cells n and n + 8.
 DEV u64 alloc_vars2(Lane &L) {
   u64 n = alloc_node(L);
@@ -271,9 +306,17 @@ DEV u64 new_node(Lane &L, u32 tag, u32 lab, u64 p1, u64 p2) {
 // Consume a binary node: read both aux words, then free the node.
 DEV void take(Lane &L, u64 p, u64 &x, u64 &y) {
   u64 a = paddr(p);
-  x = ld(*L.P, a);
-  y = ld(*L.P, a + 8);
-  st16(hw(*L.P, a & ~15ull), P_FREE, P_FREE);
+  u64 *w = hw(*L.P, a & ~15ull);
+  ld16(w, x, y);
+  st16(w, P_FREE, P_FREE);
+}
+
+// take() of two nodes, their loads in flight together.
+DEV void take2(Lane &L, u64 p, u64 q, u64 &p1, u64 &p2, u64 &q1, u64 &q2) {
+  u64 *wp = hw(*L.P, paddr(p) & ~15ull), *wq = hw(*L.P, paddr(q) & ~15ull);
+  ld16x2(wp, wq, p1, p2, q1, q2);
+  st16(wp, P_FREE, P_FREE);
+  st16(wq, P_FREE, P_FREE);
 }
 
 // Retire this lane's stores before a port to new memory becomes visible.
@@ -450,8 +493,7 @@ DEV u64 as_wire(u64 p) { return is_endpoint(p) ? wire_port(paddr(p)) : p; }
 DEV void anni2(Lane &L, u32 &np, u64 a, u64 b) {
   L.rw[RW_ANNI]++;
   u64 a1, a2, b1, b2;
-  take(L, a, a1, a2);
-  take(L, b, b1, b2);
+  take2(L, a, b, a1, a2, b1, b2);
   push_link(L, np, a1, b1);
   push_link(L, np, a2, b2);
 }
@@ -459,8 +501,7 @@ DEV void anni2(Lane &L, u32 &np, u64 a, u64 b) {
 DEV void comm22(Lane &L, u32 &np, u64 a, u64 b) {
   L.rw[RW_COMM]++;
   u64 a1, a2, b1, b2;
-  take(L, a, a1, a2);
-  take(L, b, b1, b2);
+  take2(L, a, b, a1, a2, b1, b2);
   u64 v1 = alloc_vars2(L), v3 = alloc_vars2(L);
   u64 v2 = v1 + 8, v4 = v3 + 8;
   u64 A1 = new_node(L, ptag(a), plab(a), var_port(v1), var_port(v2));
