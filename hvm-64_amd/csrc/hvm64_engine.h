@@ -117,6 +117,9 @@ struct Params {
 // MBOX_PAIRS redex pairs handed to the wave by another wave.
 static const u32 MBOX_PAIRS = 64;
 static const u32 MBOX_WORDS = 16 + 2 * MBOX_PAIRS;
+// Count word value written to every mailbox by the wave that ends the
+// reduction (S reached 0), so idle waves see the end without reading S.
+static const u64 MBOX_DONE = ~0ull;
 
 // Heap layout: the first RESERVED_WORDS words are fixed cells.
 static const u64 RESERVED_WORDS = 64;

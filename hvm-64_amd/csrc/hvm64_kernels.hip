@@ -162,14 +162,19 @@ __global__ void k_expand(Params P, u64 old_root_cell) {
 // S = (queued << 32) | active waves; the reduction is complete when S == 0,
 // a state no wave can leave.  S is one word shared by every wave and its
 // device-scope accesses all go to one memory channel, so busy waves touch it
-// only to spill (and read it every 16 iterations), and idle waves poll it
-// with a growing, jittered backoff capped near 1 ms: polling it faster slows
-// every other memory access of the reduction (measured: 2.6x slower overall
-// with a 50 us cap on bitonic sort of 2^16 keys).
+// only to spill (and read it every 16 iterations).  Idle waves poll their own
+// mailbox and read S only every IDLE_CHECK polls (for redexes on the global
+// ring); the wave that brings S to 0 writes MBOX_DONE into every mailbox, so
+// the end is seen within one poll.  Reading S more often slows every other
+// memory access of the reduction (tree-sum depth 18: 1,260 MIPS reading it
+// every 64 polls, 1,850 every 1,024; without mailboxes, a 50 us backoff cap
+// made bitonic sort of 2^16 keys 2.6x slower).  Under a reduce(limit) budget
+// mailboxes are off and idle waves poll S with a growing, jittered backoff
+// capped near 1 ms.
 static const u32 REFRESH = 16;        // busy iterations between control-word reads
 static const u32 MAX_BACKOFF = 16384;  // idle backoff cap, in s_sleep(2) units (~0.9 ms)
 static const u32 MBOX_POLL = 16;       // s_sleep(2) units between mailbox polls (~1 us)
-static const u32 IDLE_CHECK = 64;      // mailbox polls between reads of S (a power of two)
+static const u32 IDLE_CHECK = 1024;    // mailbox polls between reads of S (a power of two)
 static const u32 MBOX_KEEP = 80;       // redexes a busy wave keeps before handing some off
 static const u32 FILL_BELOW = 32;      // batched nets: pending redexes under which a busy wave claims
 
@@ -329,13 +334,20 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
         S_cache = __shfl(old, 0, 64) - 1;
         active = false;
         idle_polls = 0;
-        if (S_cache == 0) break;  // nothing active, nothing queued: done
+        if (S_cache == 0) {  // nothing active, nothing queued: done
+          // no wave is active, so no delivery is in flight: tell the idle
+          // waves through their mailboxes
+          if (use_mbox)
+            for (u32 w = lane; w < P.total_waves; w += 64) gst(P.mbox + (u64)w * MBOX_WORDS, MBOX_DONE);
+          break;
+        }
       }
       st[ST_IDLE_IT]++;
       if (use_mbox) {
         u64 k = 0;
         if (lane == 0) k = gld(my_box);
         k = __shfl(k, 0, 64);
+        if (k == MBOX_DONE) break;
         if (k > 0) {  // a delivery: the sender already counted this wave active
           if (lane < k) {
             const u64 a = gld(my_box + 16 + 2 * lane), b = gld(my_box + 16 + 2 * lane + 1);
