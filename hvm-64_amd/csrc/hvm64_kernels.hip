@@ -162,7 +162,7 @@ __global__ void k_expand(Params P, u64 old_root_cell) {
 // S = (queued << 32) | active waves; the reduction is complete when S == 0,
 // a state no wave can leave.  S is one word shared by every wave and its
 // device-scope accesses all go to one memory channel, so busy waves touch it
-// only to spill (and read it every 16 iterations).  Idle waves poll their own
+// only to spill (and read it every 64 iterations).  Idle waves poll their own
 // mailbox and read S only every IDLE_CHECK polls (for redexes on the global
 // ring); the wave that brings S to 0 writes MBOX_DONE into every mailbox, so
 // the end is seen within one poll.  Reading S more often slows every other
@@ -171,7 +171,7 @@ __global__ void k_expand(Params P, u64 old_root_cell) {
 // made bitonic sort of 2^16 keys 2.6x slower).  Under a reduce(limit) budget
 // mailboxes are off and idle waves poll S with a growing, jittered backoff
 // capped near 1 ms.
-static const u32 REFRESH = 16;        // busy iterations between control-word reads
+static const u32 REFRESH = 64;        // busy iterations between control-word reads
 static const u32 MAX_BACKOFF = 16384;  // idle backoff cap, in s_sleep(2) units (~0.9 ms)
 static const u32 MBOX_POLL = 16;       // s_sleep(2) units between mailbox polls (~1 us)
 static const u32 IDLE_CHECK = 1024;    // mailbox polls between reads of S (a power of two)
