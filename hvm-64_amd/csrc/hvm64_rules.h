@@ -773,10 +773,9 @@ DEV void exec_step(Lane &L, Interp &I, u64 w0, u64 w1, u32 &np) {
   t_set(I, c, var_port(v + 8));
 }
 
-// def.rs:217-289: start definition `di` with t0 = `t0`.
-DEV void interp_begin(Lane &L, Interp &I, u32 di, u64 t0) {
+// def.rs:217-289: start the definition whose record is `d`, with t0 = `t0`.
+DEV void interp_begin(Lane &L, Interp &I, const DefRec &d, u64 t0) {
   const Params &P = *L.P;
-  const DefRec d = load_def(L, di);
   I.Tl = HVM_TO_LDS(L.T_lds);
   I.Tg = L.T_glb;
   I.t_lds = P.t_lds && L.T_lds;
@@ -848,7 +847,7 @@ DEV void expand_call(Lane &L, Interp &I, u32 &np, u64 cell, u64 port) {
       return;
     }
     L.rw[RW_DREF]++;
-    interp_begin(L, I, def_index(oaddr), expand_ref(cell));
+    interp_begin(L, I, load_def(L, def_index(oaddr)), expand_ref(cell));
     return;
   }
   u64 out = ld(P, cell);
@@ -894,7 +893,7 @@ DEV void call(Lane &L, Interp &I, u32 &np, u64 port, u64 trg) {
     return;
   }
   L.rw[RW_DREF]++;
-  interp_begin(L, I, di, trg);
+  interp_begin(L, I, d, trg);
 }
 
 // interact.rs:6-44.  The arms are tested in the reference's order; each arm
@@ -967,7 +966,7 @@ DEV void run_task(Lane &L, bool has, u64 a, u64 b, u32 di, u64 t0) {
   I.t_lds = false;
   u32 np = 0;
   if (has) interact(L, I, np, a, b);
-  else if (di != NO_DEF) interp_begin(L, I, di, t0);
+  else if (di != NO_DEF) interp_begin(L, I, load_def(L, di), t0);
   for (;;) {
     if (I.left > 0) interp_run(L, I, np);
     if (np > 0) flush_links(L, np);
