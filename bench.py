@@ -282,7 +282,9 @@ def main():
     size, text, expected, desc = workload(args.workload, args.size, args.seed)
     heap = hvm64.Heap(int(args.heap_gb * (1 << 30)))
     heap.set_timeout(300)
-    host = hvm64.Host(text, heap)
+    t_up = time.perf_counter()
+    host = hvm64.Host(text, heap)  # parse, encode and upload the book
+    upload_ms = (time.perf_counter() - t_up) * 1e3
     main_def = host.defs["main"]
 
     def step():
@@ -304,7 +306,9 @@ def main():
         kernel_ms.append(net.last_kernel_ms())
         total += net.rwts.total()
     elapsed = time.perf_counter() - t0
+    t_rb = time.perf_counter()
     nf = host.readback(net)
+    readback_ms = (time.perf_counter() - t_rb) * 1e3
     rw = net.rwts
     stats = net.sched_stats()
     ok = nf == expected
@@ -341,6 +345,10 @@ def main():
         "rewrites": rw.as_dict(),
         "kernel_ms": round(k_ms, 3),
         "kernel_mips": round(rw.total() / k_ms / 1e3, 2),
+        # one net end to end: book upload + reset/boot/normal + readback
+        "e2e": {"book_upload_ms": round(upload_ms, 3), "step_ms": round(elapsed / args.steps * 1e3, 3),
+                "readback_ms": round(readback_ms, 3),
+                "mips": round(rw.total() / ((upload_ms + readback_ms) / 1e3 + elapsed / args.steps) / 1e6, 2)},
         "result_ok": ok,
         "sched": stats,
         "roofline": roofline,
