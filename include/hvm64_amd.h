@@ -126,7 +126,9 @@ int hvm_gpu_reduce(hvm_gpu *h, uint64_t limit, uint64_t *popped, int *is_normal)
  *   Nets booted after a reset share the handle's heap and counters.
  * hvm_gpu_normal_batch: expands every root (roots[i] is replaced by the
  *   expanded root) and reduces all booted nets to normal form in one launch.
- *   hvm_gpu_rewrites then reports the sums over the batch.
+ *   hvm_gpu_rewrites then reports the sums over the batch.  In this launch
+ *   a busy wave that runs low on redexes also claims from the global ring
+ *   (nearly every wave is busy with many nets). 
  * hvm_gpu_root_ports: the port each root leads to (a Num port for a numeric
  *   result; Var(root) while the root's wire is open).
  * hvm_gpu_readback_root: hvm_gpu_readback of one batch root. */
