@@ -321,8 +321,9 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
 
     // ---- empty rings: idle.  An idle wave is in the idle set (its bit in
     // P.idle_bits) and polls its own mailbox, where a busy wave can hand it
-    // redexes directly; every IDLE_CHECK polls it also reads S for the end
-    // of the reduction and for redexes on the global ring.
+    // redexes directly and where the end of the reduction is announced
+    // (MBOX_DONE); every IDLE_CHECK polls it also reads S, for redexes on
+    // the global ring, errors and the deadline.
     if (n == 0) {
       if (active) {
         if (use_mbox && lane == 0)
