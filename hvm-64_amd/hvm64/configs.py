@@ -153,16 +153,61 @@ def mix_choice(rng, log_uniform=False):
     return kind, size
 
 
+def split_defs(book_text):
+    """{name: body} of a book's definitions: a definition is a line
+    `@name = net` and the `& a ~ b` lines that follow it."""
+    defs, name = {}, None
+    for line in book_text.splitlines():
+        m = re.match(r"@([A-Za-z0-9_.$]+)\s*=(.*)$", line)
+        if m:
+            name = m.group(1)
+            defs[name] = [m.group(2).strip()]
+        elif line.strip() and name is not None:
+            defs[name].append(line.strip())
+    return {k: "\n".join(v) for k, v in defs.items()}
+
+
 def mix_book(variants, seed=7):
-    """One book holding each (kind, size) variant under its prefix; returns
-    (book_text, {(kind, size): (main_name, expected)})."""
-    parts, table = [], {}
+    """One book holding each (kind, size) variant; returns (book_text,
+    {(kind, size): (main_name, expected)}).
+
+    A definition whose text is the same in every size of a kind, and whose
+    references all lead to such definitions, is written once under the
+    kind's prefix (`t_` / `b_`); the others (main, and for bitonic sort the
+    key hash and the generator that calls it) are written per variant under
+    its prefix.  Every net reduces exactly as it would with its own book."""
+    table, by_kind = {}, {}
     for kind, size in sorted(set(variants)):
         text, expected = variant(kind, size, seed)
-        pre = variant_prefix(kind, size)
-        parts.append(prefixed(text, pre))
-        table[(kind, size)] = (pre + "main", expected)
-    return "\n".join(parts), table
+        by_kind.setdefault(kind, {})[size] = split_defs(text)
+        table[(kind, size)] = (variant_prefix(kind, size) + "main", expected)
+    parts = []
+    for kind, sizes in sorted(by_kind.items()):
+        books = list(sizes.values())
+        names = set(books[0])
+        shared = {n for n in names if all(b.get(n) == books[0][n] for b in books)}
+        changed = True
+        while changed:  # a shared definition may only reach shared ones
+            changed = False
+            for n in sorted(shared):
+                if any(r not in shared for r in _NAME.findall(books[0][n])):
+                    shared.discard(n)
+                    changed = True
+        kpre = variant_prefix(kind, 0)[0] + "_"
+
+        def emit(name, body, pre):
+            body = _NAME.sub(lambda m: "@" + (kpre if m.group(1) in shared else pre) + m.group(1), body)
+            lines = body.split("\n")
+            return "\n".join([f"@{pre}{name} = {lines[0]}"] + lines[1:])
+
+        for n in sorted(shared):
+            parts.append(emit(n, books[0][n], kpre))
+        for size, defs in sorted(sizes.items()):
+            pre = variant_prefix(kind, size)
+            for n in sorted(defs):
+                if n not in shared:
+                    parts.append(emit(n, defs[n], pre))
+    return "\n".join(parts) + "\n", table
 
 
 def batch_mix(n, seed=7, log_uniform=False):

@@ -48,3 +48,19 @@ def test_bitonic_tree_is_sorted(oracle, k, seed):
 def test_church(oracle, exp):
     r = oracle.run(configs.church(exp, 16), heap_words=1 << 24)
     assert r["nf"] == configs.church_expected(exp)
+
+
+def test_mix_book_shares_definitions_without_changing_results(oracle):
+    """The batched/serving book writes definitions common to every size of a
+    program once; each variant's main still reduces to the same normal form
+    with the same five counts as that variant's own book."""
+    variants = [("tree_sum", 8), ("tree_sum", 10), ("bitonic", 6), ("bitonic", 8)]
+    book, table = configs.mix_book(variants, seed=7)
+    own = sum(len(configs.split_defs(configs.variant(k, s, 7)[0])) for k, s in variants)
+    assert len(configs.split_defs(book)) < own
+    for kind, size in variants:
+        main, expected = table[(kind, size)]
+        shared = oracle.run(book, heap_words=1 << 24, entry=main)
+        alone = oracle.run(configs.variant(kind, size, 7)[0], heap_words=1 << 24)
+        assert shared["nf"] == alone["nf"] == expected
+        assert shared["rwts"] == alone["rwts"]
