@@ -45,31 +45,69 @@ def workload(name, size, seed):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(name, size, seed):
-    """Times the oracle's parallel_normal (runtime/src/parallel.rs restated)
-    with T = 2^floor(log2(cores)) threads, cores capped at 16, and its
-    single-threaded normal(), on the same workload at the same size (bitonic
-    2^16: about 360M rewrites, 10-30 s of CPU work for both runs)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# Host cores one GPU's share of the machine may use (the box's worker budget);
+# the reference's parallel_normal takes 2^floor(log2(available cores)) threads
+# (runtime/src/parallel.rs:55-57) of that share.
+CPU_SHARE = 16
+
+
+def _median(xs):
+    xs = sorted(xs)
+    return xs[len(xs) // 2]
+
+
+def cpu_baseline(name, size, seed, runs=5):
+    """Times the oracle (the reference evaluator restated in C++) on this
+    machine's host cores, median of `runs` runs each:
+
+    * parallel_normal (runtime/src/parallel.rs restated) with
+      T = 2^floor(log2(cores)) threads, cores = min(affinity, CPU_SHARE), on
+      the same workload at the same size (bitonic 2^16: ~1.7 s a run);
+    * single-threaded normal() (runtime/src/net.rs:55-60) on the same program
+      two sizes smaller (a quarter of the work), so five runs stay bounded.
+
+    Also returns the full-size run's normal form and counts (the GPU's are
+    compared with them)."""
     try:
         import oracle_lib
-        from hvm64 import configs
         o = oracle_lib.Oracle()
     except Exception as e:  # the oracle is optional for the GPU figure
-        return {"value": None, "unit": "MIPS", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
-    cores = min(16, os.cpu_count() or 1)
+        return {"value": None, "unit": "MIPS", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}, None
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = nproc
+    cores = max(1, min(affinity, CPU_SHARE))
     threads = 1
     while threads * 2 <= cores:
         threads *= 2
-    small = size
-    text = workload(name, small, seed)[1]
-    single = o.run(text, heap_words=1 << 28, threads=0)
-    multi = o.run(text, heap_words=1 << 28, threads=threads)
-    mips_1 = single["rwts"]["rwts"] / single["seconds"] / 1e6
-    mips_t = multi["rwts"]["rwts"] / multi["seconds"] / 1e6
-    return {"value": round(mips_t, 2), "unit": "MIPS", "cores": threads, "kind": "port",
-            "sample": f"{name} size {small}: {multi['rwts']['rwts']} rewrites, parallel_normal with {threads} threads "
-                      f"({multi['seconds']:.3f} s); single-thread normal() {mips_1:.2f} MIPS ({single['seconds']:.3f} s)",
-            "single_thread_mips": round(mips_1, 2)}
+    text = workload(name, size, seed)[1]
+    multi = [o.run(text, heap_words=1 << 29, threads=threads) for _ in range(runs)]
+    small = max(1, size - 2)
+    small_text = workload(name, small, seed)[1]
+    single = [o.run(small_text, heap_words=1 << 28, threads=0) for _ in range(runs)]
+    mips_t = _median([r["rwts"]["rwts"] / r["seconds"] / 1e6 for r in multi])
+    mips_1 = _median([r["rwts"]["rwts"] / r["seconds"] / 1e6 for r in single])
+    sec_t = _median([r["seconds"] for r in multi])
+    sec_1 = _median([r["seconds"] for r in single])
+    out = {"value": round(mips_t, 2), "unit": "MIPS", "cores": threads, "kind": "port",
+           "sample": f"{name} size {size}: {multi[0]['rwts']['rwts']} rewrites, parallel_normal with {threads} threads, "
+                     f"median of {runs} runs ({sec_t:.3f} s); single-thread normal() on size {small} "
+                     f"({single[0]['rwts']['rwts']} rewrites): {mips_1:.2f} MIPS, median of {runs} ({sec_1:.3f} s)",
+           "single_thread_mips": round(mips_1, 2), "runs": runs, "nproc": nproc, "affinity": affinity,
+           "cpu_share": CPU_SHARE, "cpu_model": cpu_model()}
+    return out, multi[0]
 
 
 def copy_calibration(nbytes):
@@ -354,7 +392,12 @@ def main():
         "roofline": roofline,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.workload, size, args.seed)
+        base, ref = cpu_baseline(args.workload, size, args.seed)
+        out["cpu_baseline"] = base
+        if ref is not None:
+            # the oracle's full-size run: same normal form, same five counts
+            out["oracle_match"] = ref["nf"] == nf and ref["rwts"] == rw.as_dict()
+            out["result_ok"] = ok = ok and out["oracle_match"]
     print(json.dumps(out), flush=True)
     heap.close()
     return 0 if ok else 1

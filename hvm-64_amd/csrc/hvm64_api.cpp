@@ -71,6 +71,9 @@ namespace {
 
 thread_local std::string g_open_error;
 
+// First word of a book image written by hvm_book_encode: "HVM64BK1".
+const u64 BOOK_IMAGE_MAGIC = 0x314B4234364D5648ull;
+
 void hip_check(hipError_t e, const char *what) {
   if (e != hipSuccess) throw Error(HVM_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -276,10 +279,14 @@ u64 *batch_buffer(hvm_gpu *h, u64 words) {
   return h->d_batch;
 }
 
+// Every entry point runs with the handle's device current: allocations,
+// copies and launches made from a thread other than the one that opened the
+// handle must land on the handle's GPU.
 template <class F>
 int guarded(hvm_gpu *h, F fn) {
   if (!h) return HVM_E_ARG;
   try {
+    hip_check(hipSetDevice(h->device), "hipSetDevice");
     fn();
     return 0;
   } catch (Error &e) {
@@ -369,6 +376,7 @@ int hvm_gpu_open_grid(int device, uint64_t heap_bytes, uint32_t max_blocks, hvm_
 
 void hvm_gpu_close(hvm_gpu *h) {
   if (!h) return;
+  (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   dfree(h->heap);
   dfree(h->alloc_pos);
@@ -410,6 +418,12 @@ int hvm_gpu_load_book_text(hvm_gpu *h, const char *text) {
 int hvm_gpu_load_book(hvm_gpu *h, const hvm_book_blob *book) {
   return guarded(h, [&] {
     if (!book) throw Error(HVM_E_ARG, "null book");
+    if (book->n_defs && !book->defs) throw Error(HVM_E_ARG, "null definition array");
+    for (u32 i = 0; i < book->n_defs; i++) {
+      const hvm_def_blob &d = book->defs[i];
+      if ((d.n_lab_words && !d.lab_bits) || (d.n_instr && !d.instr))
+        throw Error(HVM_E_ARG, "definition " + std::to_string(i) + " has a null array");
+    }
     std::vector<std::string> names;
     std::vector<hvm64::LabSet> labs;
     std::vector<hvm64::DefCode> code;
@@ -453,6 +467,48 @@ int hvm_gpu_load_book(hvm_gpu *h, const hvm_book_blob *book) {
     }
     upload_book(h);
   });
+}
+
+int hvm_gpu_load_book_image(hvm_gpu *h, const void *image, uint64_t len) {
+  std::vector<hvm_def_blob> defs;
+  std::vector<std::string> names;
+  int rc = guarded(h, [&] {
+    if (!image) throw Error(HVM_E_ARG, "null image");
+    const u64 *w = (const u64 *)image;
+    const u64 nw = len / 8;
+    if (len % 8 || nw < 2 || w[0] != BOOK_IMAGE_MAGIC) throw Error(HVM_E_ARG, "not a book image");
+    const u64 n = w[1];
+    u64 pos = 2;
+    auto need = [&](u64 words) {
+      if (words > nw || pos > nw - words) throw Error(HVM_E_ARG, "truncated book image");
+    };
+    defs.resize(n);
+    names.resize(n);
+    for (u64 i = 0; i < n; i++) {
+      need(3);
+      const u32 name_len = (u32)w[pos], n_instr = (u32)(w[pos] >> 32);
+      const u32 n_slots = (u32)w[pos + 1], min_safe = (u32)(w[pos + 1] >> 32);
+      const u32 n_lab = (u32)w[pos + 2];
+      pos += 3;
+      const u64 name_words = ((u64)name_len + 7) / 8;
+      need(name_words + n_lab + 2 * (u64)n_instr);
+      names[i].assign((const char *)(w + pos), name_len);
+      pos += name_words;
+      hvm_def_blob &d = defs[i];
+      d.min_safe = min_safe;
+      d.n_lab_words = n_lab;
+      d.lab_bits = w + pos;
+      pos += n_lab;
+      d.n_instr = n_instr;
+      d.instr = (const hvm_instr *)(w + pos);
+      pos += 2 * (u64)n_instr;
+      d.n_slots = n_slots;
+    }
+    for (u64 i = 0; i < n; i++) defs[i].name = names[i].c_str();
+  });
+  if (rc) return rc;
+  hvm_book_blob book{(uint32_t)defs.size(), defs.data()};
+  return hvm_gpu_load_book(h, &book);
 }
 
 int hvm_gpu_def_id(hvm_gpu *h, const char *name, uint32_t *id) {
@@ -593,8 +649,11 @@ int hvm_gpu_reduce(hvm_gpu *h, uint64_t limit, uint64_t *popped, int *is_normal)
       return;
     }
     run_reduce(h, limit);
-    if (popped) *popped = h->last.rw[RW_POPS] - before;
-    if (is_normal) *is_normal = (h->last.S >> 32) == 0;
+    const u64 n = h->last.rw[RW_POPS] - before;
+    if (popped) *popped = n;
+    // net.rs:41-52 returns None as soon as `limit` redexes were popped, even
+    // when none is left, and Some(count) only when the queue ran dry first
+    if (is_normal) *is_normal = n < limit && (h->last.S >> 32) == 0;
   });
 }
 
@@ -683,6 +742,44 @@ static int host_text(const std::function<std::string()> &fn, char *buf, uint64_t
 int hvm_book_print(const char *book_text, char *buf, uint64_t cap, uint64_t *len) {
   if (!book_text) return HVM_E_ARG;
   return host_text([&] { return hvm64::show_book(hvm64::parse_book(book_text)); }, buf, cap, len);
+}
+
+int hvm_book_encode(const char *book_text, void *buf, uint64_t cap, uint64_t *len) {
+  if (!book_text) return HVM_E_ARG;
+  try {
+    std::vector<u64> img;
+    hvm64::with_big_stack([&] {
+      hvm64::Book book = hvm64::parse_book(book_text);
+      hvm64::Host host;
+      host.insert_book(book);
+      img.push_back(BOOK_IMAGE_MAGIC);
+      img.push_back(host.ndefs());
+      for (u32 i = 0; i < host.ndefs(); i++) {
+        const std::string &name = host.def_name(i);
+        const hvm64::DefCode &c = host.code(i);
+        const hvm64::LabSet &l = host.labs(i);
+        img.push_back((u64)name.size() | ((u64)c.ninstr() << 32));
+        img.push_back((u64)c.nslots | ((u64)l.min_safe << 32));
+        img.push_back((u64)l.bits.size());
+        std::vector<u64> nw((name.size() + 7) / 8, 0);
+        std::memcpy(nw.data(), name.data(), name.size());
+        img.insert(img.end(), nw.begin(), nw.end());
+        img.insert(img.end(), l.bits.begin(), l.bits.end());
+        img.insert(img.end(), c.instr.begin(), c.instr.end());
+      }
+    });
+    const u64 bytes = img.size() * 8;
+    if (len) *len = bytes;
+    if (!buf || cap < bytes) return HVM_E_TRUNC;
+    std::memcpy(buf, img.data(), bytes);
+    return 0;
+  } catch (Error &e) {
+    g_open_error = e.what();
+    return e.code;
+  } catch (std::exception &e) {
+    g_open_error = e.what();
+    return HVM_E_ARG;
+  }
 }
 
 int hvm_book_label_sets(const char *book_text, char *buf, uint64_t cap, uint64_t *len) {

@@ -88,11 +88,14 @@ DEV u64 ld(const Params &P, u64 addr) { return HVM_LD(hw(P, addr)); }
 // two.  The compiler does not count this store in its waits; since the
 // counter retires in order, an uncounted store only makes later waits
 // longer, never shorter, and a full wait (publish) covers it.
+// gfx940 and later need two wait states before a VALU instruction may
+// overwrite the data registers of a store wider than 64 bits; the compiler
+// does not insert them after inline asm, so the asm ends with `s_nop 1`.
 DEV void st16(u64 *p, u64 w0, u64 w1) {
 #if defined(__HIP_DEVICE_COMPILE__)
   typedef u64 u64x2 __attribute__((ext_vector_type(2)));
   const u64x2 d = {w0, w1};
-  asm volatile("global_store_dwordx4 %0, %1, off sc1" ::"v"(p), "v"(d) : "memory");
+  asm volatile("global_store_dwordx4 %0, %1, off sc1\n\ts_nop 1" ::"v"(p), "v"(d) : "memory");
 #else
   HVM_ST(p, w0);
   HVM_ST(p + 1, w1);

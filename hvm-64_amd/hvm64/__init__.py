@@ -33,8 +33,8 @@ EXPORTS = [
     "hvm_gpu_reduce", "hvm_gpu_rewrites", "hvm_gpu_shortcuts", "hvm_gpu_readback", "hvm_gpu_last_kernel_ms",
     "hvm_gpu_grid", "hvm_gpu_set_timeout", "hvm_gpu_sched_stats", "hvm_gpu_open_grid",
     "hvm_gpu_boot_batch", "hvm_gpu_normal_batch", "hvm_gpu_root_ports", "hvm_gpu_readback_root",
-    "hvm_gpu_snapshot",
-    "hvm_book_print", "hvm_book_label_sets", "hvm_net_apply_tree",
+    "hvm_gpu_snapshot", "hvm_gpu_load_book_image",
+    "hvm_book_print", "hvm_book_label_sets", "hvm_net_apply_tree", "hvm_book_encode",
 ]
 
 _lib = None
@@ -43,6 +43,23 @@ _lib = None
 class NetImage(ctypes.Structure):
     """hvm_net_image: root var cell, port address of heap word 0, heap words."""
     _fields_ = [("root_cell", ctypes.c_uint64), ("heap_base", ctypes.c_uint64), ("heap_words", ctypes.c_uint64)]
+
+
+class Instr(ctypes.Structure):
+    """hvm_instr: one device instruction (two words)."""
+    _fields_ = [("w0", ctypes.c_uint64), ("w1", ctypes.c_uint64)]
+
+
+class DefBlob(ctypes.Structure):
+    """hvm_def_blob: one encoded definition."""
+    _fields_ = [("name", ctypes.c_char_p), ("min_safe", ctypes.c_uint32), ("n_lab_words", ctypes.c_uint32),
+                ("lab_bits", ctypes.POINTER(ctypes.c_uint64)), ("n_instr", ctypes.c_uint32),
+                ("instr", ctypes.POINTER(Instr)), ("n_slots", ctypes.c_uint32)]
+
+
+class BookBlob(ctypes.Structure):
+    """hvm_book_blob: an encoded book."""
+    _fields_ = [("n_defs", ctypes.c_uint32), ("defs", ctypes.POINTER(DefBlob))]
 
 
 class HvmError(RuntimeError):
@@ -89,6 +106,9 @@ def load_library(path=None):
     lib.hvm_gpu_root_ports.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(u64), ctypes.POINTER(u64)]
     lib.hvm_gpu_readback_root.argtypes = [P, u64, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
     lib.hvm_gpu_snapshot.argtypes = [P, ctypes.POINTER(NetImage), u64, u64, ctypes.POINTER(u64)]
+    lib.hvm_gpu_load_book.argtypes = [P, ctypes.POINTER(BookBlob)]
+    lib.hvm_gpu_load_book_image.argtypes = [P, ctypes.c_char_p, u64]
+    lib.hvm_book_encode.argtypes = [ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
     for name in ("hvm_book_print", "hvm_book_label_sets"):
         getattr(lib, name).argtypes = [ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
     lib.hvm_net_apply_tree.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
@@ -193,12 +213,78 @@ class _Defs:
             return None
 
 
-class Host:
-    """Host::new(&book) (host/src/host.rs:25): loads a book into the device."""
+_IMAGES = {}
 
-    def __init__(self, book_text, heap):
+
+def encode_book(book_text):
+    """Book::from_str + Host::insert_book into a book image (bytes), the
+    flat encoded form hvm_gpu_load_book_image takes.  Images are cached by
+    the book text's hash for the life of the process."""
+    import hashlib
+    key = hashlib.sha256(book_text.encode()).hexdigest()
+    img = _IMAGES.get(key)
+    if img is not None:
+        return img
+    lib = load_library()
+    n = ctypes.c_uint64()
+    rc = lib.hvm_book_encode(book_text.encode(), None, 0, ctypes.byref(n))
+    if rc not in (0, 9):
+        raise HvmError(rc, lib.hvm_gpu_last_error(None).decode())
+    buf = ctypes.create_string_buffer(n.value)
+    rc = lib.hvm_book_encode(book_text.encode(), buf, n.value, ctypes.byref(n))
+    if rc != 0:
+        raise HvmError(rc, lib.hvm_gpu_last_error(None).decode())
+    img = buf.raw[:n.value]
+    _IMAGES[key] = img
+    return img
+
+
+def image_blobs(image):
+    """The hvm_book_blob structures describing a book image: what a host
+    that encodes books itself (the Rust Host's Def<InterpretedDef> list)
+    passes to hvm_gpu_load_book.  Returns (BookBlob, keepalive)."""
+    import struct
+    words = struct.unpack(f"<{len(image) // 8}Q", image)
+    assert words[0] == 0x314B4234364D5648, "not a book image"
+    n, pos = words[1], 2
+    defs = (DefBlob * max(n, 1))()
+    keep = [defs]
+    for i in range(n):
+        name_len, n_instr = words[pos] & 0xFFFFFFFF, words[pos] >> 32
+        n_slots, min_safe = words[pos + 1] & 0xFFFFFFFF, words[pos + 1] >> 32
+        n_lab = words[pos + 2]
+        pos += 3
+        name = image[pos * 8:pos * 8 + name_len]
+        pos += (name_len + 7) // 8
+        labs = (ctypes.c_uint64 * max(n_lab, 1))(*words[pos:pos + n_lab])
+        pos += n_lab
+        ins = (Instr * max(n_instr, 1))(*[Instr(words[pos + 2 * k], words[pos + 2 * k + 1]) for k in range(n_instr)])
+        pos += 2 * n_instr
+        keep += [name, labs, ins]
+        defs[i] = DefBlob(name, min_safe, n_lab, labs, n_instr, ins, n_slots)
+    return BookBlob(n, defs), keep
+
+
+class Host:
+    """Host::new(&book) (host/src/host.rs:25): loads a book into the device.
+
+    `via`: "text" parses and encodes the book on every load, "image" loads
+    the cached book image (encode_book), "blob" passes the image's
+    definitions as hvm_def_blob structures (hvm_gpu_load_book)."""
+
+    def __init__(self, book_text, heap, via="text"):
         self.heap = heap
-        heap._check(heap._lib.hvm_gpu_load_book_text(heap._h, book_text.encode()))
+        if via == "text":
+            heap._check(heap._lib.hvm_gpu_load_book_text(heap._h, book_text.encode()))
+        elif via == "image":
+            img = encode_book(book_text)
+            heap._check(heap._lib.hvm_gpu_load_book_image(heap._h, img, len(img)))
+        elif via == "blob":
+            blob, keep = image_blobs(encode_book(book_text))
+            heap._check(heap._lib.hvm_gpu_load_book(heap._h, ctypes.byref(blob)))
+            del keep
+        else:
+            raise ValueError(via)
         self.defs = _Defs(self)
 
     def readback(self, net):
@@ -249,11 +335,13 @@ class Net:
         self.heap._check(self.heap._lib.hvm_gpu_normal(self.heap._h))
 
     def reduce(self, limit):
-        """Returns the number of redexes popped if the net is now normal, else None."""
+        """Returns the number of redexes popped if the net is now normal, else
+        None; `last_popped` holds the count in both cases."""
         popped = ctypes.c_uint64()
         normal = ctypes.c_int()
         h = self.heap
         h._check(h._lib.hvm_gpu_reduce(h._h, limit, ctypes.byref(popped), ctypes.byref(normal)))
+        self.last_popped = popped.value
         return popped.value if normal.value else None
 
     @property

@@ -22,21 +22,24 @@ import random
 import re
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_GOLDEN = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "tests", "golden", "programs")
+# The two reference programs the workloads are built from (sum_tree.hvm and
+# bitonic_sort_lam.hvm of HigherOrderCO/hvm-64, Apache-2.0), shipped with
+# the package.
+_PROGRAMS = os.path.join(_HERE, "programs")
 
 U24 = 1 << 24
 HASH_M1 = 0x5D2B93
 HASH_M2 = 0x3C6EF3
 
 
-def _fixture(rel):
-    with open(os.path.join(_GOLDEN, rel)) as f:
+def _program(name):
+    with open(os.path.join(_PROGRAMS, name)) as f:
         return f.read()
 
 
 def tree_sum(depth):
     """Book whose @main sums a complete binary tree of 2**depth ones."""
-    text = _fixture("tests/programs/stress_tests/sum_tree.hvm")
+    text = _program("sum_tree.hvm")
     text, n = re.subn(r"@gen ~ \(24 b\)", f"@gen ~ ({int(depth)} b)", text)
     assert n == 1
     return text
@@ -69,7 +72,7 @@ def bitonic(k, seed=1, result="sum"):
 
     result="sum": the u24 sum of the sorted keys; result="tree": the sorted
     Scott-encoded tree itself (for sortedness checks at small k)."""
-    text = _fixture("examples/sort/bitonic/bitonic_sort_lam.hvm")
+    text = _program("bitonic_sort_lam.hvm")
     text, n1 = re.subn(r"@gen\$S0 = \(a b\)\n& @Leaf ~ \(a b\)\n",
                        "@gen$S0 = (a b)\n& @Leaf ~ (h b)\n& @hash ~ (a h)\n", text)
     text, n2 = re.subn(r"& @gen ~ \(10 \(0 d\)\)", f"& @gen ~ ({int(k)} (0 d))", text)
@@ -185,7 +188,8 @@ def mix_book(variants, seed=7):
     for kind, sizes in sorted(by_kind.items()):
         books = list(sizes.values())
         names = set(books[0])
-        shared = {n for n in names if all(b.get(n) == books[0][n] for b in books)}
+        # main is always per variant: the table names it by variant prefix
+        shared = {n for n in names if n != "main" and all(b.get(n) == books[0][n] for b in books)}
         changed = True
         while changed:  # a shared definition may only reach shared ones
             changed = False

@@ -92,6 +92,16 @@ int hvm_gpu_load_book_text(hvm_gpu *h, const char *book_text);
  * the Rust Host's Def<InterpretedDef> list serialised into hvm_def_blob). */
 int hvm_gpu_load_book(hvm_gpu *h, const hvm_book_blob *book);
 
+/* hvm_gpu_load_book for a book image written by hvm_book_encode (a flat,
+ * self-contained byte buffer that a host can cache instead of parsing and
+ * encoding the book's text again).  The image format: 8-byte words, "HVM64BK1",
+ * the definition count, then per definition: name length | instruction
+ * count << 32, trg registers | min_safe << 32, label-bit words, the name
+ * (zero-padded to 8 bytes), the label bits, the instructions (2 words each,
+ * hvm_instr).  No counterpart in the reference, whose Host is built from the
+ * text every time (host/src/host.rs:25-29). */
+int hvm_gpu_load_book_image(hvm_gpu *h, const void *image, uint64_t len);
+
 /* ~ Host::defs[name] (host/src/host.rs:19): index of a loaded definition. */
 int hvm_gpu_def_id(hvm_gpu *h, const char *name, uint32_t *id);
 
@@ -113,8 +123,9 @@ int hvm_gpu_encode_net_text(hvm_gpu *h, const char *net_text);
 int hvm_gpu_normal(hvm_gpu *h);
 
 /* ~ Net::reduce(limit) (runtime/src/net.rs:41-52): pops at most `limit`
- * redexes.  *is_normal = 1 when no redex is left (Some(count)), else 0
- * (None); *popped = redexes popped. */
+ * redexes; *popped = redexes popped.  As in the reference, *is_normal = 1
+ * (Some(count)) only when the redexes ran out before `limit` were popped,
+ * else 0 (None). */
 int hvm_gpu_reduce(hvm_gpu *h, uint64_t limit, uint64_t *popped, int *is_normal);
 
 /* Batches of independent nets in one reduction.  No counterpart in the
@@ -189,6 +200,11 @@ int hvm_gpu_sched_stats(hvm_gpu *h, uint64_t out[20]);
 
 /* ~ Book::from_str + Display (ast/src/parser.rs:178-183, ast/src/ast.rs:455-466). */
 int hvm_book_print(const char *book_text, char *buf, uint64_t cap, uint64_t *len);
+
+/* ~ Book::from_str + Host::insert_book (ast/src/parser.rs:178-183,
+ * host/src/host.rs:41-73) into a book image for hvm_gpu_load_book_image;
+ * *len receives the image size in bytes (HVM_E_TRUNC if `cap` is smaller). */
+int hvm_book_encode(const char *book_text, void *buf, uint64_t cap, uint64_t *len);
 
 /* ~ calculate_label_sets (host/src/calc_labels.rs:78-93): one line per
  * definition, "name: l1 l2 ...". */

@@ -64,3 +64,25 @@ def test_mix_book_shares_definitions_without_changing_results(oracle):
         alone = oracle.run(configs.variant(kind, size, 7)[0], heap_words=1 << 24)
         assert shared["nf"] == alone["nf"] == expected
         assert shared["rwts"] == alone["rwts"]
+
+
+@pytest.mark.parametrize("variants", [[("tree_sum", 10)], [("bitonic", 7)], [("tree_sum", 9), ("bitonic", 6)]])
+def test_mix_book_with_one_size_per_kind(oracle, variants):
+    """A kind present at a single size still names its main by the variant
+    prefix that the table gives."""
+    book, table = configs.mix_book(variants, seed=7)
+    defs = configs.split_defs(book)
+    for kind, size in variants:
+        main, expected = table[(kind, size)]
+        assert main in defs
+        assert oracle.run(book, heap_words=1 << 24, entry=main)["nf"] == expected
+
+
+@pytest.mark.parametrize("n,seed", [(1, 7), (8, 7), (3, 11)])
+def test_batch_mix_small(oracle, n, seed):
+    book, nets, table = configs.batch_mix(n, seed)
+    defs = configs.split_defs(book)
+    for v in set(nets):
+        main, expected = table[v]
+        assert main in defs
+        assert oracle.run(book, heap_words=1 << 24, entry=main)["nf"] == expected

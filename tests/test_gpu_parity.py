@@ -300,3 +300,194 @@ def test_snapshot_image_matches_readback(gpu_heap, text, expected):
         return cache[w]
 
     assert _image_text(img, read, img.root_cell | 1) == expected
+
+
+# ---------------------------------------------------------------------------
+# F24 `&` `|` `^` (num/src/num.rs:243-248) on the device
+# ---------------------------------------------------------------------------
+def _f24_gpu_items(heap, cases):
+    import f24_transcendental as ft
+    nf, rw, _ = run_gpu(heap, ft.book([c[:3] for c in cases]))
+    items = ft.tuple_items(nf)
+    assert len(items) == len(cases)
+    assert rw["oper"] == len(cases)
+    return items
+
+
+def test_f24_transcendental_identities_gpu(gpu_heap, oracle):
+    """Results fixed by mathematics (x^1 = x, x^0 = 1, 2^k, log_b(b^k) = k,
+    atan2 special values), rounded to f24, on the device's atan2f / logf /
+    powf."""
+    import f24_transcendental as ft
+    cases = ft.identity_cases()
+    items = _f24_gpu_items(gpu_heap, cases)
+    for (op, a, b, want), got in zip(cases, items):
+        assert got == oracle.num_display(ft.f24_raw(want)), (op, a, b, want, got)
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_f24_transcendental_random_gpu(gpu_heap, oracle, seed):
+    """Random operands whose exact result is away from an f24 rounding
+    boundary: the device's f24 result is the exact result's, and so is the
+    oracle's (the same book reduced on the CPU gives the same text)."""
+    import f24_transcendental as ft
+    cases = ft.random_cases(seed=seed, n=90)
+    items = _f24_gpu_items(gpu_heap, cases)
+    for (op, a, b, v), got in zip(cases, items):
+        assert got == oracle.num_display(ft.f24_raw(v)), (op, a, b, v, got)
+    ref = oracle.run(ft.book([c[:3] for c in cases]), heap_words=1 << 20)
+    assert ft.tuple_items(ref["nf"]) == items
+
+
+def test_f24_transcendental_nan_gpu(gpu_heap):
+    import f24_transcendental as ft
+    items = _f24_gpu_items(gpu_heap, ft.nan_cases())
+    assert all(x.lstrip("+-") == "NaN" for x in items), items
+
+
+# ---------------------------------------------------------------------------
+# ERA shortcuts of do_op / do_op_num / do_switch (instruction.rs:144-146,
+# 163-165, 194-197): books where the called definition's pattern reaches an
+# eraser that is already in the argument when the call runs
+# ---------------------------------------------------------------------------
+SHORTCUT_BOOKS = [
+    ("@main = r\n& @f ~ (* r)\n@f = ($([+1] y) y)\n", (0, 1, 0)),
+    ("@main = r\n& @f ~ (* r)\n@f = ($(a y) (a y))\n", (1, 0, 0)),
+    ("@main = r\n& @f ~ (* (5 r))\n@f = ($(a y) (a y))\n", (1, 0, 0)),
+    ("@main = r\n& @f ~ (* r)\n@f = (?(a b) (a b))\n", (0, 0, 1)),
+    ("@main = r\n& @f ~ (* (1 r))\n@f = (?((a b) c) (a (b c)))\n", (0, 0, 1)),
+    # the eraser reaches the argument through a wire linked before the call
+    ("@main = r\n& @f ~ (x r)\n& * ~ x\n@f = ($([*3] y) y)\n", (0, 1, 0)),
+    ("@main = r\n& @f ~ (x (y r))\n& * ~ x\n& * ~ y\n@f = (?(a b) ($(c d) (a (b (c d)))))\n", (1, 0, 1)),
+]
+
+
+@pytest.mark.parametrize("text,shortcuts", SHORTCUT_BOOKS)
+def test_era_shortcuts_fire(gpu_heap, oracle, text, shortcuts):
+    print("case shortcut", repr(text), flush=True)
+    nf, rw, net = run_gpu(gpu_heap, text)
+    ref = oracle.run(text, heap_words=1 << 16)
+    assert ref["shortcuts"] == shortcuts
+    assert nf == ref["nf"]
+    assert rw == ref["rwts"]
+    assert net.shortcuts == ref["shortcuts"]
+
+
+# ---------------------------------------------------------------------------
+# book loading through the encoded forms (hvm_gpu_load_book /
+# hvm_gpu_load_book_image), as a host that encodes books itself would
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("via", ["image", "blob"])
+@pytest.mark.parametrize("name", ["examples::sort::bitonic::bitonic_sort_lam", "tests::programs::f24",
+                                  "examples::arithmetic", "tests::programs::stress_tests::sum_tree"])
+def test_encoded_book_paths(gpu_heap, via, name):
+    import hvm64
+    snap = SNAPS.get(name) or next(v for k, v in SNAPS.items() if k.endswith(name.split("::")[-1]))
+    if snap["rwts"]["rwts"] > 200_000_000:
+        from hvm64 import configs
+        text = configs.tree_sum(12)
+        ref_nf = configs.tree_sum_expected(12)
+        ref_rw = None
+    else:
+        text, ref_nf, ref_rw = snap["text"], snap["normal_form"], snap["rwts"]
+    host = hvm64.Host(text, gpu_heap, via=via)
+    net = hvm64.Net(gpu_heap)
+    net.boot(host.defs["main"])
+    net.normal()
+    assert host.readback(net) == ref_nf
+    if ref_rw is not None:
+        assert net.rwts.as_dict() == ref_rw
+
+
+# ---------------------------------------------------------------------------
+# full-size configurations against the oracle
+# ---------------------------------------------------------------------------
+def test_bitonic_full_size_counts(oracle):
+    """Bitonic sort of 2^16 keys (the single-net benchmark): normal form and
+    all five counts equal the oracle's."""
+    import hvm64
+    from hvm64 import configs
+    print("case bitonic 2^16 seed 7", flush=True)
+    heap = hvm64.Heap(64 << 30)
+    try:
+        heap.set_timeout(120)
+        text = configs.bitonic(16, 7)
+        rw, nf, _ = hvm64.normal_with(text, heap=heap)
+    finally:
+        heap.close()
+    ref = oracle.run(text, heap_words=1 << 29)
+    assert nf == ref["nf"] == configs.bitonic_expected(16, 7)
+    assert rw.as_dict() == ref["rwts"]
+
+
+def test_church_full_size(oracle):
+    """Church c2^c22 applied to +1 on 0 with 16 DUP labels (a sequential
+    chain of about 4M increments): result 2^22 and the oracle's counts."""
+    import hvm64
+    from hvm64 import configs
+    print("case church c2^c22", flush=True)
+    heap = hvm64.Heap(32 << 30)
+    try:
+        heap.set_timeout(110)
+        text = configs.church(22, 16)
+        rw, nf, _ = hvm64.normal_with(text, heap=heap)
+    finally:
+        heap.close()
+    ref = oracle.run(text, heap_words=1 << 28)
+    assert nf == ref["nf"] == configs.church_expected(22)
+    assert rw.as_dict() == ref["rwts"]
+
+
+def test_batch_4096_against_oracle(oracle):
+    """The batched configuration at full size: 4,096 nets of the mix in one
+    launch; every result is its variant's, and the five counts equal the sum
+    of the oracle's per-variant counts."""
+    import hvm64
+    from hvm64 import configs
+    print("case batch 4096 seed 7", flush=True)
+    book, nets, table = configs.batch_mix(4096, seed=7)
+    heap = hvm64.Heap(64 << 30)
+    try:
+        heap.set_timeout(120)
+        host = hvm64.Host(book, heap)
+        batch = hvm64.Batch(heap)
+        batch.boot([host.defs[table[v][0]] for v in nets])
+        batch.normal()
+        ports = batch.ports()
+        got = batch.rwts.as_dict()
+    finally:
+        heap.close()
+    ref = {}
+    for v in table:
+        r = oracle.run(book, entry=table[v][0], heap_words=1 << 27)
+        assert r["nf"] == table[v][1]
+        ref[v] = r["rwts"]
+    for i, v in enumerate(nets):
+        assert ports[i] == hvm64.u24_port(int(table[v][1])), (i, v)
+    total = {k: sum(ref[v][k] for v in nets) for k in got}
+    assert got == total
+
+
+@pytest.mark.parametrize("limit", [1, 7, 1000, 50_000])
+def test_reduce_limit_bounds_pops(gpu_heap, oracle, limit):
+    """Net::reduce(limit) (net.rs:41-52) pops at most `limit` redexes per
+    call; repeated calls reach the normal form with the oracle's counts."""
+    import hvm64
+    from hvm64 import configs
+    text = configs.bitonic(7, 3)
+    host = hvm64.Host(text, gpu_heap)
+    net = hvm64.Net(gpu_heap)
+    net.boot(host.defs["main"])
+    calls = 0
+    while True:
+        r = net.reduce(limit)
+        assert net.last_popped <= limit
+        calls += 1
+        if r is not None:
+            break
+        assert net.last_popped == limit
+        assert calls < 2_000_000 // limit + 10
+    net.normal()
+    ref = oracle.run(text, heap_words=1 << 24)
+    assert host.readback(net) == ref["nf"]
+    assert net.rwts.as_dict() == ref["rwts"]

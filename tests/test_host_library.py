@@ -71,3 +71,32 @@ def test_missing_reference_is_reported():
     with pytest.raises(hvm64.HvmError) as e:
         hvm64.label_sets("@main = @nope")
     assert e.value.kind == "NOREF"
+
+
+@pytest.mark.parametrize("name", ["examples::sort::bitonic::bitonic_sort_lam", "tests::programs::f24",
+                                  "examples::arithmetic"])
+def test_book_image_round_trip(oracle, name):
+    """hvm_book_encode writes every definition of the book (Host::insert_book,
+    host.rs:41-73) into one image; the blob view of it has the definitions,
+    label sets and instruction counts the text describes."""
+    import hvm64
+    text = SNAPS[name]["text"]
+    img = hvm64.encode_book(text)
+    assert img[:8] == b"HVM64BK1" and len(img) % 8 == 0
+    blob, keep = hvm64.image_blobs(img)
+    names = [blob.defs[i].name.decode() for i in range(blob.n_defs)]
+    assert names == sorted(oracle.label_sets(text))
+    labs = hvm64.label_sets(text)
+    for i, n in enumerate(names):
+        d = blob.defs[i]
+        bits = [d.lab_bits[k] for k in range(d.n_lab_words)]
+        got = [l for l in range(d.min_safe) if l // 64 < len(bits) and (bits[l // 64] >> (l % 64)) & 1]
+        assert got == labs[n], n
+        assert d.n_instr > 0 and d.n_slots > 0
+    assert hvm64.encode_book(text) is img  # cached by the text's hash
+
+
+def test_book_image_rejects_garbage():
+    import hvm64
+    lib = hvm64.load_library()
+    assert lib.hvm_book_encode(b"@main = (", None, 0, None) == 1

@@ -2,8 +2,9 @@
 written independently here with numpy: u24 arithmetic mod 2**24, i24 two's
 complement with truncating division, f24 = IEEE f32 op then round-to-nearest-
 even of the low 8 mantissa bits (num.rs:81-90), saturating casts with NaN->0.
-The F24 `&`, `|`, `^` operators (atan2 / log / powf) are not pinned here:
-nothing in the reference fixes their last bit."""
+The F24 `&`, `|`, `^` operators (atan2 / log / powf) are pinned by the
+identities and boundary-free random cases of f24_transcendental.py: nothing
+in the reference's tests fixes their last bit, mathematics fixes these."""
 import math
 import struct
 
@@ -199,3 +200,38 @@ def test_f24_display_matches_rust_debug(oracle):
     for v in vals:
         raw = oracle.new_f24(v)
         assert oracle.num_display(raw) == rust_f32_debug(f24_val(raw)), v
+
+
+def test_f24_transcendental_identities(oracle):
+    """`x ^ 1 = x`, `x ^ 0 = 1`, `2 ^ k`, `log_b(b ^ k) = k`,
+    `atan2(+-0, x) = +-0`, `atan2(y, 0) = +-pi/2`, ... rounded to f24."""
+    import f24_transcendental as ft
+    for op, a, b, want in ft.identity_cases():
+        got = oracle.operate_binary(oracle.new_f24(a), ft.OPS[op], oracle.new_f24(b))
+        assert got == ft.f24_raw(want), (op, a, b, hex(got), hex(ft.f24_raw(want)))
+
+
+def test_f24_transcendental_random(oracle):
+    """Random f24 operands whose exact result is away from an f24 rounding
+    boundary round to the exact result's f24 value."""
+    import f24_transcendental as ft
+    for op, a, b, v in ft.random_cases(seed=5, n=600):
+        got = oracle.operate_binary(oracle.new_f24(a), ft.OPS[op], oracle.new_f24(b))
+        assert got == ft.f24_raw(v), (op, a, b, v, hex(got))
+
+
+def test_f24_transcendental_nan(oracle):
+    import f24_transcendental as ft
+    for op, a, b in ft.nan_cases():
+        got = oracle.operate_binary(oracle.new_f24(a), ft.OPS[op], oracle.new_f24(b))
+        assert (got >> 3) & 0x1F == F24 and math.isnan(f24_val(got)), (op, a, b, hex(got))
+
+
+def test_f24_transcendental_book(oracle):
+    """The book the GPU test reduces: its operands parse to the intended f24
+    values and the oracle's normal form lists the pinned results in order."""
+    import f24_transcendental as ft
+    cases = ft.identity_cases()[:40]
+    nf = oracle.run(ft.book([(op, a, b) for op, a, b, _ in cases]), heap_words=1 << 20)["nf"]
+    items = ft.tuple_items(nf)
+    assert items == [oracle.num_display(ft.f24_raw(w)) for _, _, _, w in cases]
