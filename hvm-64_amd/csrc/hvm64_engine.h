@@ -6,7 +6,7 @@
 
 namespace hvm {
 
-// Instruction kinds (runtime/src/instruction.rs:324-362).  On the device an
+// Instruction kinds (runtime/src/instruction.rs:34-72).  On the device an
 // instruction is two words:
 //   w0 = kind | a<<16 | b<<32 | c<<48      (a, b, c: trg register slots)
 //   w1 = Const/LinkConst: port; Ctr: label; Op: op; OpNum: op | rhs<<32;

@@ -300,7 +300,7 @@ u32 parse_tree(Arena &arena, const std::string &text) {
 }
 
 // ---------------------------------------------------------------------------
-// printer (ast.rs:455-497), iterative
+// printer (ast.rs:221-263), iterative
 // ---------------------------------------------------------------------------
 static void ctr_brackets(u32 lab, std::string &open, const char *&close) {
   if (lab == 0) open = "(";
@@ -370,7 +370,7 @@ std::string show_book(const Book &book) {
   return s;
 }
 
-// apply_tree (ast.rs:346-357) with ensure_no_conflicts (ast.rs:403-410).
+// apply_tree (ast.rs:112-123) with ensure_no_conflicts (ast.rs:125-132, 169-176).
 static bool var_index(const std::string &s, u64 &out) {  // util/src/create_var.rs:21-33
   u64 n = 0;
   for (char ch : s) {
@@ -510,7 +510,7 @@ std::map<std::string, LabSet> calculate_label_sets(const Book &book) {
 }
 
 // ---------------------------------------------------------------------------
-// encoder (encode.rs:94-248) + register compaction
+// encoder (encode.rs:38-163) + register compaction
 // ---------------------------------------------------------------------------
 namespace {
 struct RawInstr {
@@ -536,7 +536,7 @@ struct Enc {
     tree(n.root, t);
     for (auto &r : n.redexes) redex(r.first, r.second);
   }
-  void redex(u32 a, u32 b) {  // encode.rs:137-152
+  void redex(u32 a, u32 b) {  // encode.rs:52-66
     const TNode &x = A.nodes[a], &y = A.nodes[b];
     u32 nil, other;
     if (x.kind == TK::Era) nil = a, other = b;
@@ -557,7 +557,7 @@ struct Enc {
     emit(I_CONST, t, 0, 0, 0, nil_port(A.nodes[nil]));
     tree(other, t);
   }
-  void tree(u32 id, u32 t) {  // encode.rs:153-189
+  void tree(u32 id, u32 t) {  // encode.rs:68-104
     const TNode &n = A.nodes[id];
     switch (n.kind) {
       case TK::Era:
@@ -750,7 +750,7 @@ u32 Host::max_slots() const {
 }
 
 // ---------------------------------------------------------------------------
-// readback (readback.rs:288-365) from the device's preorder stream
+// readback (readback.rs:23-87) from the device's preorder stream
 // ---------------------------------------------------------------------------
 std::string readback_text(const Host &host, const std::vector<u64> &stream, size_t &pos,
                           std::map<u64, u64> &vars, u64 &next_var) {

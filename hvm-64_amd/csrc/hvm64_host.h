@@ -73,7 +73,7 @@ std::string show_book(const Book &book);
 std::string show_num(u32 raw);
 std::string var_name(u64 id);  // util/src/create_var.rs:5-15
 
-// Net::apply_tree (ast.rs:346-357).
+// Net::apply_tree (ast.rs:112-123).
 void apply_tree(Arena &arena, Net &net, u32 arg);
 
 // ---------------------------------------------------------------------------
@@ -105,7 +105,7 @@ class Host {
   const LabSet &labs(u32 id) const { return labs_[id]; }
   const DefCode &code(u32 id) const { return code_[id]; }
   // Compile an AST net into instructions whose register 0 is the net's root
-  // (encode.rs:98-104 for definitions, encode.rs:116-120 for encode_net).
+  // (encode.rs:13-19 for definitions, encode.rs:31-35 for encode_net).
   DefCode encode(const Arena &arena, const Net &net) const;
 
   // Flattened device book.
@@ -128,7 +128,7 @@ std::map<std::string, LabSet> calculate_label_sets(const Book &book);
 
 // Rebuild normal-form text from the device's preorder serialisation
 // (k_readback): principal ports in preorder, Var(cell) for aux-aux wires.
-// Variable names follow first occurrence (readback.rs:330-339).
+// Variable names follow first occurrence (readback.rs:53-62).
 std::string readback_text(const Host &host, const std::vector<u64> &stream, size_t &pos,
                           std::map<u64, u64> &vars, u64 &next_var);
 

@@ -2,8 +2,8 @@
 //
 // Hot path: runtime/src/net.rs:41-60 (Net::reduce / Net::normal) with the
 // rules of runtime/src/interact.rs, the instruction interpreter of
-// runtime/src/def.rs:217-289 and instruction.rs:400-500, and ExpandDef
-// (net.rs:335-380).
+// runtime/src/def.rs:217-289 and instruction.rs:110-210, and ExpandDef
+// (net.rs:72-109).
 //
 // Memory model.  Heap words are shared by every wave on all 8 XCDs, so every
 // heap access is a device-scope operation: relaxed agent-scope loads/stores
@@ -46,7 +46,7 @@ __global__ void k_fill(u64 *words, u64 n, u64 value) {
 
 // Run one definition's instructions with t0 = `t0` on lane 0 of wave 0:
 // Net::boot (net.rs:30-32, counts one dref) or Host::encode_net
-// (encode.rs:116-120, counts nothing).  Redexes go to the global ring.
+// (encode.rs:31-35, counts nothing).  Redexes go to the global ring.
 __global__ void k_exec(Params P, u32 def, u64 t0, u32 count_dref) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   __shared__ u64 s_pl[2 * LINK_CAP * 64];
@@ -659,7 +659,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   }
 }
 
-// Serialise the tree hanging from `root_cell` in preorder (readback.rs:288-365
+// Serialise the tree hanging from `root_cell` in preorder (readback.rs:23-87
 // walk order: aux port 1 before aux port 2).  Principal ports are emitted as
 // they are; a wire between two aux ports is emitted as Var(cell) of the var
 // cell that joins them.  g->result = number of entries, or ERR_READBACK.

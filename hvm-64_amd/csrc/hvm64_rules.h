@@ -111,7 +111,7 @@ DEV void publish() { HVM_PUBLISH(); }
 DEV u64 var_port(u64 cell) { return mkport(T_VAR, 0, cell); }
 DEV u64 wire_port(u64 cell) { return mkport(T_RED, 0, cell); }
 // Endpoints are stored as Var ports; the Red tag only marks trg registers
-// that the interpreter may resolve (the reference's Trg::wire, linker.rs:352-405).
+// that the interpreter may resolve (the reference's Trg::wire, linker.rs:233-281).
 DEV u64 norm_endpoint(u64 p) { return is_endpoint(p) ? (paddr(p) | T_VAR) : p; }
 
 // ---------------------------------------------------------------------------
@@ -376,7 +376,7 @@ DEV u32 redex_class(u64 a, u64 b, const lds_u64 *book, u32 ndefs) {
   return CLS_OTHER;
 }
 
-// linker.rs:194-209: skippable pairs count as an erasure and are dropped.
+// linker.rs:70-85: skippable pairs count as an erasure and are dropped.
 DEV void redux(Lane &L, u64 a, u64 b) {
   if (is_skippable(a) && is_skippable(b)) {
     L.rw[RW_ERAS]++;
@@ -470,7 +470,7 @@ DEV void flush_links(Lane &L, u32 &np) {
 }
 
 // Follow a wire endpoint whose other side has already linked (the
-// reference's Trg::target on a wire, linker.rs:401-404).  The caller owns the
+// reference's Trg::target on a wire, linker.rs:278-280).  The caller owns the
 // endpoint, so once a cell holds a port the cell is ours to free.
 DEV u64 resolve(Lane &L, u64 p) {
   const Params &P = *L.P;
@@ -587,7 +587,7 @@ DEV bool labs_has(const Lane &L, const DefRec &d, u32 lab) {  // def.rs:30-37
   return (w >> (lab & 63)) & 1;
 }
 
-// Definition instructions (def.rs:217-289, instruction.rs:390-493).
+// Definition instructions (def.rs:217-289, instruction.rs:110-210).
 //
 // A definition runs as a sequence of steps with the same shape for every
 // instruction kind, so that lanes running different definitions stay
@@ -636,11 +636,11 @@ DEV void alloc2(Lane &L, u64 &n0, u64 &n1) {
   n1 = alloc_node(L);
 }
 
-// The rare instruction kinds (Op, OpNum, Switch; instruction.rs:425-493).
+// The rare instruction kinds (Op, OpNum, Switch; instruction.rs:134-203).
 DEV void exec_num(Lane &L, Interp &I, u32 kind, u32 b, u32 c, u64 w1, u64 port, u32 &np) {
   const Params &P = *L.P;
   const u32 lab = (u32)(w1 & 0xFFFF);
-  if (kind == I_OPNUM) {  // instruction.rs:445-462
+  if (kind == I_OPNUM) {  // instruction.rs:154-172
     const u32 rhs = (u32)(w1 >> 32);
     if (ptag(port) == T_NUM) {
       L.rw[RW_OPER]++;
@@ -662,7 +662,7 @@ DEV void exec_num(Lane &L, Interp &I, u32 kind, u32 b, u32 c, u64 w1, u64 port, 
     }
     return;
   }
-  if (kind == I_OP) {  // instruction.rs:425-442
+  if (kind == I_OP) {  // instruction.rs:134-152
     if (ptag(port) == T_NUM) {
       u64 v, n;
       alloc2(L, v, n);
@@ -687,7 +687,7 @@ DEV void exec_num(Lane &L, Interp &I, u32 kind, u32 b, u32 c, u64 w1, u64 port, 
     }
     return;
   }
-  // I_SWITCH, instruction.rs:465-493
+  // I_SWITCH, instruction.rs:174-203
   if (ptag(port) == T_NUM) {
     L.rw[RW_OPER]++;
     const u32 n = pnum(port);
@@ -723,7 +723,7 @@ DEV void exec_num(Lane &L, Interp &I, u32 kind, u32 b, u32 c, u64 w1, u64 port, 
   }
 }
 
-// One instruction (instruction.rs:390-493).  The common kinds come first
+// One instruction (instruction.rs:110-210).  The common kinds come first
 // and take short paths: a wave executes the union of its lanes' paths, one
 // instruction every four cycles.
 DEV void exec_step(Lane &L, Interp &I, u64 w0, u64 w1, u32 &np) {
@@ -758,7 +758,7 @@ DEV void exec_step(Lane &L, Interp &I, u64 w0, u64 w1, u32 &np) {
     return;
   }
   const u32 lab = (u32)(w1 & 0xFFFF);
-  if (ptag(port) == T_CTR && plab(port) == lab) {  // do_ctr's fast path (instruction.rs:406-410)
+  if (ptag(port) == T_CTR && plab(port) == lab) {  // do_ctr's fast path (instruction.rs:116-121)
     u64 x, y;
     take(L, port, x, y);
     L.rw[RW_ANNI]++;
@@ -766,7 +766,7 @@ DEV void exec_step(Lane &L, Interp &I, u64 w0, u64 w1, u32 &np) {
     t_set(I, c, as_wire(y));
     return;
   }
-  u64 v, n;  // instruction.rs:411-421
+  u64 v, n;  // instruction.rs:126-131
   alloc2(L, v, n);
   st2_new(P, v, P_NONE, P_NONE);
   st2_new(P, n, var_port(v), var_port(v + 8));
@@ -831,7 +831,7 @@ DEV void interp_run(Lane &L, Interp &I, u32 &np) {
 
 DEV u64 expand_ref(u64 cell) { return mkport(T_REF, 0xFFFF, cell); }
 
-// net.rs:353-380.  `cell` holds the ExpandDef's `out` endpoint.
+// net.rs:82-109.  `cell` holds the ExpandDef's `out` endpoint.
 DEV void expand_call(Lane &L, Interp &I, u32 &np, u64 cell, u64 port) {
   const Params &P = *L.P;
   if (ptag(port) == T_REF && port != P_ERA) {

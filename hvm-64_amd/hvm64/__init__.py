@@ -316,7 +316,7 @@ class Host:
             return buf.value.decode()
 
     def encode_net(self, net, net_text):
-        """Host::encode_net(net, Var(root), &ast) (host/src/encode.rs:116-120)."""
+        """Host::encode_net(net, Var(root), &ast) (host/src/encode.rs:31-35)."""
         h = self.heap
         h._check(h._lib.hvm_gpu_encode_net_text(h._h, net_text.encode()))
 
@@ -433,7 +433,7 @@ def _host_text(fn, *args):
 
 
 def book_print(book_text):
-    """Book::from_str + Display (ast/src/parser.rs:178-183, ast/src/ast.rs:455-466)."""
+    """Book::from_str + Display (ast/src/parser.rs:178-183, ast/src/ast.rs:221-232)."""
     return _host_text(load_library().hvm_book_print, book_text)
 
 
@@ -447,7 +447,7 @@ def label_sets(book_text):
 
 
 def apply_tree(net_text, tree_text):
-    """Net::apply_tree (ast/src/ast.rs:346-357), as net text."""
+    """Net::apply_tree (ast/src/ast.rs:112-123), as net text."""
     return _host_text(load_library().hvm_net_apply_tree, net_text, tree_text)
 
 

@@ -31,7 +31,7 @@ extern "C" {
 #define HVM_E_OOM 2          /* heap exhausted (panic "OOM", runtime/src/allocator.rs:104) */
 #define HVM_E_UNIMPL_PAIR 3  /* Switch-Ctr(0) / Op-Switch (unimplemented!, runtime/src/interact.rs:39-42) */
 #define HVM_E_DIV0 4         /* integer / or % by zero (Rust panic, num/src/num.rs:183-212) */
-#define HVM_E_UNBOUND 5      /* unbound variable (host/src/encode.rs:131) */
+#define HVM_E_UNBOUND 5      /* unbound variable (host/src/encode.rs:46) */
 #define HVM_E_NOREF 6        /* reference to a missing definition (host/src/host.rs:35) */
 #define HVM_E_QUEUE 7        /* global redex ring overflow */
 #define HVM_E_READBACK 8     /* malformed net during readback */
@@ -43,7 +43,7 @@ extern "C" {
 typedef struct hvm_gpu hvm_gpu;
 
 /* One interpreted-definition instruction in device form
- * (runtime/src/instruction.rs:324-362):
+ * (runtime/src/instruction.rs:34-72):
  *   w0 = kind | a<<16 | b<<32 | c<<48   kind: 0 Const, 1 Link, 2 LinkConst,
  *        3 Ctr, 4 Op, 5 OpNum, 6 Switch, 7 Wires; a, b, c: trg registers
  *   w1 = Const/LinkConst: port; Ctr: label; Op: op tag; OpNum: op | rhs<<32;
@@ -115,7 +115,7 @@ int hvm_gpu_reset(hvm_gpu *h);
 int hvm_gpu_boot(hvm_gpu *h, uint32_t def_id);
 
 /* ~ Host::encode_net(net, Trg::port(Var(root)), &ast_net)
- * (host/src/encode.rs:116-120): writes a net given as text into the root. */
+ * (host/src/encode.rs:31-35): writes a net given as text into the root. */
 int hvm_gpu_encode_net_text(hvm_gpu *h, const char *net_text);
 
 /* ~ Net::normal() (runtime/src/net.rs:55-60): expand, then reduce to normal
@@ -156,7 +156,7 @@ int hvm_gpu_rewrites(hvm_gpu *h, uint64_t out[5]);
 int hvm_gpu_shortcuts(hvm_gpu *h, uint64_t out[3]);
 
 /* ~ Host::readback(&net) + Display (host/src/readback.rs:23-33,
- * ast/src/ast.rs:468-478): normal form as text.  If `cap` is too small,
+ * ast/src/ast.rs:234-244): normal form as text.  If `cap` is too small,
  * returns HVM_E_TRUNC with the needed length (without NUL) in *len. */
 int hvm_gpu_readback(hvm_gpu *h, char *buf, uint64_t cap, uint64_t *len);
 
@@ -198,7 +198,7 @@ int hvm_gpu_sched_stats(hvm_gpu *h, uint64_t out[20]);
  * hvm_gpu_readback's buffer convention; errors are described by
  * hvm_gpu_last_error(NULL). */
 
-/* ~ Book::from_str + Display (ast/src/parser.rs:178-183, ast/src/ast.rs:455-466). */
+/* ~ Book::from_str + Display (ast/src/parser.rs:178-183, ast/src/ast.rs:221-232). */
 int hvm_book_print(const char *book_text, char *buf, uint64_t cap, uint64_t *len);
 
 /* ~ Book::from_str + Host::insert_book (ast/src/parser.rs:178-183,
@@ -210,7 +210,7 @@ int hvm_book_encode(const char *book_text, void *buf, uint64_t cap, uint64_t *le
  * definition, "name: l1 l2 ...". */
 int hvm_book_label_sets(const char *book_text, char *buf, uint64_t cap, uint64_t *len);
 
-/* ~ Net::apply_tree (ast/src/ast.rs:346-357): `net_text` applied to
+/* ~ Net::apply_tree (ast/src/ast.rs:112-123): `net_text` applied to
  * `tree_text`, printed as a net. */
 int hvm_net_apply_tree(const char *net_text, const char *tree_text, char *buf, uint64_t cap, uint64_t *len);
 

@@ -301,7 +301,7 @@ struct Tree {
   Tree() = default;
   Tree(Tree &&) = default;
   Tree &operator=(Tree &&) = default;
-  ~Tree() {  // iterative drop (ast.rs:429-453 avoids stack overflow the same way)
+  ~Tree() {  // iterative drop (ast.rs:195-219 avoids stack overflow the same way)
     std::vector<std::unique_ptr<Tree>> stack;
     if (a) stack.push_back(std::move(a));
     if (b) stack.push_back(std::move(b));
@@ -353,7 +353,7 @@ static bool var_to_num(const std::string &s, size_t &out) {  // create_var.rs:21
   return true;
 }
 
-static void print_tree(std::string &o, const Tree &t) {  // ast.rs:480-497
+static void print_tree(std::string &o, const Tree &t) {  // ast.rs:246-263
   switch (t.kind) {
     case Tree::Era: o += '*'; break;
     case Tree::CtrK: {
@@ -569,7 +569,7 @@ static Book parse_book(const std::string &s) { return parse_eof(s, [](Parser &p)
 static AstNet parse_net(const std::string &s) { return parse_eof(s, [](Parser &p) { return p.parse_net(); }); }
 static Tree parse_tree(const std::string &s) { return parse_eof(s, [](Parser &p) { return p.parse_tree(); }); }
 
-static void tree_fresh(const Tree &t, size_t &fresh) {  // ast.rs:403-410
+static void tree_fresh(const Tree &t, size_t &fresh) {  // ast.rs:169-176
   if (t.kind == Tree::VarK) {
     size_t n;
     if (var_to_num(t.name, n)) fresh = std::max(fresh, n);
@@ -578,7 +578,7 @@ static void tree_fresh(const Tree &t, size_t &fresh) {  // ast.rs:403-410
   if (t.b) tree_fresh(*t.b, fresh);
 }
 
-static void apply_tree(AstNet &net, Tree arg) {  // ast.rs:346-357
+static void apply_tree(AstNet &net, Tree arg) {  // ast.rs:112-123
   size_t fresh = 0;
   tree_fresh(net.root, fresh);
   for (auto &r : net.redexes) { tree_fresh(r.first, fresh); tree_fresh(r.second, fresh); }
@@ -663,7 +663,7 @@ struct Wire {
 };
 static inline Wire port_wire(Port p) { return Wire{p.addr()}; }
 
-// Trg: a port or a wire; wires reuse the Red tag (linker.rs:352-405).
+// Trg: a port or a wire; wires reuse the Red tag (linker.rs:233-281).
 struct Trg {
   uint64_t v;
   static Trg port(Port p) { return Trg{p.v}; }
@@ -720,7 +720,7 @@ struct LabSet {  // def.rs:7-62
 };
 
 enum InstrKind : uint8_t { I_CONST, I_LINK, I_LINKCONST, I_CTR, I_OP, I_OPNUM, I_SWITCH, I_WIRES };
-struct Instruction {  // instruction.rs:324-362
+struct Instruction {  // instruction.rs:34-72
   InstrKind kind;
   uint32_t a = 0, b = 0, c = 0, d = 0;  // TrgId indices (meaning depends on kind)
   Lab lab = 0;                           // Ctr label / Op op
@@ -739,7 +739,7 @@ struct alignas(32) Def {  // def.rs:86-95
   LabSet labs;
   int kind = 0;  // 0 interpreted, 1 expand
   InterpretedDef data;
-  Port out{0};  // ExpandDef payload (net.rs:343-345)
+  Port out{0};  // ExpandDef payload (net.rs:72-74)
 };
 static inline Port new_ref(const Def *d) { return Port::make(Ref, d->labs.min_safe, (uint64_t)d); }
 static inline Def *port_def(Port p) { return reinterpret_cast<Def *>(p.addr()); }
@@ -764,7 +764,7 @@ struct Heap {
   ~Heap() {}
 };
 
-struct RedexQueue {  // linker.rs:431-468
+struct RedexQueue {  // linker.rs:307-344
   std::vector<std::pair<Port, Port>> fast, slow;
   bool pop(Port &a, Port &b) {
     if (!fast.empty()) { a = fast.back().first; b = fast.back().second; fast.pop_back(); return true; }
@@ -775,7 +775,7 @@ struct RedexQueue {  // linker.rs:431-468
   bool empty() const { return fast.empty() && slow.empty(); }
 };
 
-static bool redex_would_shrink(Port a, Port b) {  // linker.rs:470-475
+static bool redex_would_shrink(Port a, Port b) {  // linker.rs:347-351
   return a == ERA || b == ERA || (a.tag() == Ctr && b.tag() == Ctr && a.lab() == b.lab()) ||
          (!(a.tag() == Ref || b.tag() == Ref) && (a.tag() == NumT || b.tag() == NumT));
 }
@@ -1121,7 +1121,7 @@ struct Net {
     }
   }
 
-  // --- instruction.rs:403-500
+  // --- instruction.rs:110-210
   std::pair<Trg, Trg> do_ctr(Lab lab, Trg trg) {
     Port port = trg.target();
     if (port.tag() == Ctr && port.lab() == lab) {
@@ -1194,7 +1194,7 @@ struct Net {
     return {Trg::port(m.p1), Trg::port(m.p2)};
   }
 
-  // --- net.rs:335-380  (ExpandDef)
+  // --- net.rs:72-109  (ExpandDef)
   Port new_expand(Port out) {
     Def *d = new Def();
     d->labs = LabSet::ALL();
@@ -1293,10 +1293,10 @@ void Net::parallel_normal(size_t cores) {
     auto n = std::make_unique<Net>(areas.back().get(), false);
     n->root = root;
     n->next = next > heap_start ? next - heap_start : 0;
-    n->head = 0;  // the fresh net's own (empty) free list, as parallel.rs:286 reads it
+    n->head = 0;  // the fresh net's own (empty) free list, as parallel.rs:25 reads it
     n->tid = tid;
     n->tids = tids;
-    size_t count = all.size() / (tids - tid);  // parallel.rs:290 divides the original length
+    size_t count = all.size() / (tids - tid);  // parallel.rs:29 divides the original length
     for (size_t k = 0; k < count && taken < all.size(); k++, taken++) n->redux(all[taken].first, all[taken].second);
     nets.push_back(std::move(n));
   }
@@ -1463,7 +1463,7 @@ struct Host {
     return new_ref(it->second.get());
   }
 
-  // Encoder over InterpretedDef (encode.rs:204-248)
+  // Encoder over InterpretedDef (encode.rs:119-163)
   struct DefEnc {
     InterpretedDef &d;
     using T = uint32_t;
@@ -1500,7 +1500,7 @@ struct Host {
       return {av, aw, bv, bw};
     }
   };
-  // Encoder over a live Net (encode.rs:250-277)
+  // Encoder over a live Net (encode.rs:165-192)
   struct NetEnc {
     Net &n;
     using T = Trg;
@@ -1518,7 +1518,7 @@ struct Host {
     }
   };
   template <class E>
-  struct EncState {  // encode.rs:123-190
+  struct EncState {  // encode.rs:38-104
     const Host &host;
     E &enc;
     std::map<std::string, typename E::T> scope;
@@ -1535,7 +1535,7 @@ struct Host {
         if (x.kind == Tree::NumK) { p = Port::num(x.num); return true; }
         return false;
       };
-      // (Era, t) | (t, Era) first, then Ref, then Num (encode.rs:138-142)
+      // (Era, t) | (t, Era) first, then Ref, then Num (encode.rs:52-66)
       if (a.kind == Tree::Era) { port = ERA; t = &b; }
       else if (b.kind == Tree::Era) { port = ERA; t = &a; }
       else if (a.kind == Tree::RefK) { nil(a, port); t = &b; }
@@ -1621,7 +1621,7 @@ struct Host {
     st.finish();
   }
 
-  // readback.rs:288-365
+  // readback.rs:23-87
   struct RB {
     const Host &host;
     std::unordered_map<uint64_t, size_t> vars;

@@ -52,7 +52,7 @@ def test_label_sets_reference_vector():
 
 
 def test_apply_tree_text():
-    """ast.rs:346-357: `x & ...` becomes `y & ... & x ~ (arg y)`, y fresh."""
+    """ast.rs:112-123: `x & ...` becomes `y & ... & x ~ (arg y)`, y fresh."""
     import hvm64
     assert hvm64.apply_tree("(a a)", "(a a)") == "b\n  & (a a) ~ ((a a) b)"
     assert hvm64.apply_tree("@main", "64") == "b\n  & @main ~ (64 b)"
