@@ -168,6 +168,7 @@ def bench_batch(args):
     heap = hvm64.Heap(int(args.heap_gb * (1 << 30)))
     heap.set_timeout(300)
     host = hvm64.Host(book, heap)
+    n_compiled = heap.set_compiled(not args.interpreted)
     defs = [host.defs[table[v][0]] for v in nets]
     expected = [hvm64.u24_port(int(table[v][1])) for v in nets]
 
@@ -218,7 +219,8 @@ def bench_batch(args):
         "data": "synthetic (seeded mix of in-net generated trees and keys)",
         "config": {"workload": f"{args.nets} independent nets in one launch: tree-sum depth U[8,16] or bitonic "
                                f"2^k keys k U[6,12] (seed {args.seed})",
-                   "nets": args.nets, "seed": args.seed, "heap_gb": args.heap_gb, "grid_blocks": blocks, "waves": waves},
+                   "nets": args.nets, "seed": args.seed, "heap_gb": args.heap_gb, "grid_blocks": blocks, "waves": waves,
+                   "definitions": "interpreted" if args.interpreted else f"compiled ({n_compiled})"},
         "rewrites_per_step": rw.total(),
         "rewrites": rw.as_dict(),
         "kernel_ms": round(sum(kernel_ms) / len(kernel_ms), 3),
@@ -298,6 +300,8 @@ def main():
     ap.add_argument("--heap-gb", type=float, default=None,
                     help="heap size (default 192 GiB of the MI355X's 288 GB; serve: split over the slots)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--interpreted", action="store_true",
+                    help="run every definition interpreted (compiled definitions off)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before the steps, copy a buffer of this many MiB on the device (a known-size "
                          "kernel that calibrates HBM byte counters in a profile)")
@@ -322,6 +326,7 @@ def main():
     heap.set_timeout(300)
     t_up = time.perf_counter()
     host = hvm64.Host(text, heap)  # parse, encode and upload the book
+    n_compiled = heap.set_compiled(not args.interpreted)
     upload_ms = (time.perf_counter() - t_up) * 1e3
     main_def = host.defs["main"]
 
@@ -378,7 +383,8 @@ def main():
         "dtype": "u64",
         "data": "synthetic (seeded in-net key generator)",
         "config": {"workload": desc, "size": size, "seed": args.seed, "heap_gb": args.heap_gb,
-                   "grid_blocks": blocks, "waves": waves},
+                   "grid_blocks": blocks, "waves": waves,
+                   "definitions": "interpreted" if args.interpreted else f"compiled ({n_compiled})"},
         "rewrites_per_step": rw.total(),
         "rewrites": rw.as_dict(),
         "kernel_ms": round(k_ms, 3),

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libhvm64_amd.so")
 
-SOURCES = ["hvm64_kernels.hip", "hvm64_host.cpp", "hvm64_api.cpp"]
+SOURCES = ["hvm64_kernels.hip", "hvm64_host.cpp", "hvm64_defcomp.cpp", "hvm64_api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950",

@@ -43,6 +43,10 @@ struct hvm_gpu {
   DefRec *d_defs = nullptr;
   u64 *d_instrs = nullptr;
   u64 *d_labbits = nullptr;
+  u64 *d_cdefs = nullptr;  // compiled-definition records
+  u64 *d_tmpl = nullptr;   // templates of the compiled definitions
+  u32 dev_instr_words = 0, tmpl_words = 0, n_compiled = 0;
+  bool compiled = true;    // calls run compiled definitions where there are some
   u32 ndefs = 0;
   u64 *d_trg = nullptr;
   u32 trg_slots = 0;
@@ -103,13 +107,18 @@ Params make_params(hvm_gpu *h, u64 limit) {
   P.ndefs = h->ndefs;
   P.trg_slots = h->trg_slots;
   P.total_waves = h->waves;
-  P.instr_words = (u32)h->instrs_host.size();
+  P.instr_words = h->dev_instr_words;
   P.lab_words = (u32)h->labbits_host.size();
+  P.cdefs = h->d_cdefs;
+  P.tmpl = h->d_tmpl;
+  P.tmpl_words = h->tmpl_words;
+  P.compiled = h->compiled ? 1 : 0;
   P.idle_bits = h->idle_bits;
   P.mbox = h->mbox;
   P.idle_words = h->idle_words;
   P.t_lds = h->max_nslots <= reduce_lds_slots() ? 1 : 0;
-  P.book_lds = (u64)h->ndefs * 4 + P.instr_words + P.lab_words <= reduce_book_lds_words() ? 1 : 0;
+  P.book_lds =
+      (u64)h->ndefs * 5 + P.instr_words + P.lab_words + P.tmpl_words <= reduce_book_lds_words() ? 1 : 0;
   P.limit = limit;
   P.timeout_ticks = h->timeout_ticks;
   return P;
@@ -167,16 +176,16 @@ void validate_book(const hvm_gpu *h) {
   }
 }
 
-// Call ring group of a definition by instruction count: the lanes of a call
-// iteration wait for the longest definition among them.
-u32 call_group(u32 ilen) { return ilen <= 4 ? 0 : ilen <= 8 ? 1 : ilen <= 16 ? 2 : 3; }
-
 void upload_book(hvm_gpu *h) {
   validate_book(h);
-  for (auto &d : h->defs_host) d.group = call_group(d.ilen);
+  // The device book: the definitions' own instructions, then their compiled
+  // programs and templates (hvm64_defcomp.cpp).
+  hvm64::DeviceBook b = hvm64::build_device_book(h->defs_host, h->instrs_host);
   dfree(h->d_defs);
   dfree(h->d_instrs);
   dfree(h->d_labbits);
+  dfree(h->d_cdefs);
+  dfree(h->d_tmpl);
   h->ndefs = (u32)h->defs_host.size();
   auto up = [&](auto *&dst, const auto &vec) {
     size_t bytes = std::max<size_t>(vec.size() * sizeof(vec[0]), 16);
@@ -184,12 +193,15 @@ void upload_book(hvm_gpu *h) {
     if (!vec.empty()) hip_check(hipMemcpy(dst, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice), "upload book");
   };
   up(h->d_defs, h->defs_host);
-  up(h->d_instrs, h->instrs_host);
+  up(h->d_instrs, b.instrs);
   up(h->d_labbits, h->labbits_host);
-  u32 slots = 1;
-  for (auto &d : h->defs_host) slots = std::max(slots, d.nslots);
-  h->max_nslots = slots;
-  ensure_trg(h, slots);
+  up(h->d_cdefs, b.cdefs);
+  up(h->d_tmpl, b.tmpl);
+  h->dev_instr_words = (u32)b.instrs.size();
+  h->tmpl_words = (u32)b.tmpl.size();
+  h->n_compiled = b.n_compiled;
+  h->max_nslots = b.max_slots;
+  ensure_trg(h, b.max_slots);
 }
 
 // Net::new.  The heap is filled with FREE once, when the handle opens; a
@@ -388,6 +400,8 @@ void hvm_gpu_close(hvm_gpu *h) {
   dfree(h->d_defs);
   dfree(h->d_instrs);
   dfree(h->d_labbits);
+  dfree(h->d_cdefs);
+  dfree(h->d_tmpl);
   dfree(h->d_trg);
   dfree(h->d_rb_out);
   dfree(h->d_rb_stack);
@@ -704,6 +718,13 @@ int hvm_gpu_readback(hvm_gpu *h, char *buf, uint64_t cap, uint64_t *len) {
 
 int hvm_gpu_last_kernel_ms(hvm_gpu *h, double *ms) {
   return guarded(h, [&] { *ms = h->last_ms; });
+}
+
+int hvm_gpu_set_compiled(hvm_gpu *h, int on, uint32_t *n_compiled) {
+  return guarded(h, [&] {
+    h->compiled = on != 0;
+    if (n_compiled) *n_compiled = h->n_compiled;
+  });
 }
 
 int hvm_gpu_set_timeout(hvm_gpu *h, double seconds) {

@@ -11,11 +11,46 @@ namespace hvm {
 //   w0 = kind | a<<16 | b<<32 | c<<48      (a, b, c: trg register slots)
 //   w1 = Const/LinkConst: port; Ctr: label; Op: op; OpNum: op | rhs<<32;
 //        Wires: d (fourth slot)
-enum : u32 { I_CONST = 0, I_LINK, I_LINKCONST, I_CTR, I_OP, I_OPNUM, I_SWITCH, I_WIRES };
+enum : u32 { I_CONST = 0, I_LINK, I_LINKCONST, I_CTR, I_OP, I_OPNUM, I_SWITCH, I_WIRES,
+              // kinds that only compiled definitions use (hvm64_defcomp.cpp):
+              I_TEMPLATE,  // w1 = word offset of the definition's template in the template table
+              I_LINKT,     // link register a with a template object; w1 = template spec
+              I_KINDS };
+// w0 bit 8: the operand register holds a port to be read as a wire (the
+// compiled form of a trg that reached a runtime value through a var cell)
+static const u32 I_ASWIRE = 1u << 8;
 
 HVM_HD u64 instr_w0(u32 kind, u32 a, u32 b, u32 c) {
   return (u64)kind | ((u64)a << 16) | ((u64)b << 32) | ((u64)c << 48);
 }
+
+// Compiled definitions.  A definition compiled at book load is a residual
+// program (the instructions whose operands depend on the port that met the
+// definition, run as interpreted instructions) and a template: heap slots
+// whose words are known up to the addresses of the slots and the values of
+// registers (written by one I_TEMPLATE instruction), redexes between them,
+// and the rewrite counts of the instructions evaluated at load time.
+//
+// Template word specs are port-shaped: Ref / Num ports are literals; a
+// binary tag (Op, Switch, Ctr) with slot s in the address field is the
+// principal port of the node in slot s; a Var port with 2s+h in the address
+// field is the var cell in half h of slot s; P_NONE and P_FREE are literals;
+// any other port with the Red tag and r in the address field is the value of
+// register r (as a var endpoint).
+//
+// Template layout: word 0 = slots | redexes << 16; word 1 = the eight
+// counts (anni, comm, eras, dref, oper, and the three ERA shortcuts) as
+// bytes; then two specs per slot, then two specs per redex.
+// Compiled-definition record (one word per definition): program offset
+// (instruction index, 24 bits) | program length << 24 (16 bits) |
+// registers << 40 (8 bits) | template slots << 48 (8 bits); 0 = not compiled.
+HVM_HD u32 cdef_ioff(u64 c) { return (u32)(c & 0xFFFFFF); }
+HVM_HD u32 cdef_ilen(u64 c) { return (u32)((c >> 24) & 0xFFFF); }
+HVM_HD u32 cdef_nslots(u64 c) { return (u32)((c >> 40) & 0xFF); }
+HVM_HD u32 cdef_tslots(u64 c) { return (u32)((c >> 48) & 0xFF); }
+// Template slot j sits TSLOT_STRIDE bytes after slot j - 1 (consecutive
+// positions of a lane's allocation slice).
+static const u64 TSLOT_STRIDE = 64 * 16;
 
 // One book definition (def.rs:86-95 + InterpretedDef, def.rs:194-199).
 struct DefRec {
@@ -105,6 +140,10 @@ struct Params {
   u32 instr_words;     // words in `instrs`
   u32 lab_words;       // words in `labbits`
   u32 t_lds;           // 1: every definition's trg registers fit in LDS
+  const u64 *cdefs;    // compiled-definition records (one word per definition)
+  const u64 *tmpl;     // template table of the compiled definitions
+  u32 tmpl_words;      // words in `tmpl`
+  u32 compiled;        // 1: calls run the compiled form where there is one
   u64 *idle_bits;      // one bit per wave of the reduce grid: the wave is idle
   u64 *mbox;           // per-wave mailboxes (MBOX_WORDS each)
   u32 idle_words;      // words in idle_bits

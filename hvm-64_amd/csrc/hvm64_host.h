@@ -123,6 +123,28 @@ class Host {
   std::vector<DefCode> code_;
 };
 
+// A definition compiled into a residual program and a template
+// (hvm64_defcomp.cpp).  ok = false: the definition runs interpreted.
+struct CompiledDef {
+  bool ok = false;
+  std::vector<u64> instr;  // residual program, device format (2 words each)
+  u32 nslots = 1;          // registers of the residual program
+  u32 tslots = 0;          // heap slots of the template
+  std::vector<u64> tmpl;   // template words (empty: no I_TEMPLATE)
+  std::string why;         // ok = false: the reason
+};
+CompiledDef compile_def(const u64 *instr, u32 ninstr, u32 nslots);
+
+// The book as the device reads it: the definitions' own instructions, then
+// the compiled programs; one compiled-definition record per definition (0:
+// interpreted); the template table.  Sets each definition's call ring group.
+struct DeviceBook {
+  std::vector<u64> instrs, cdefs, tmpl;
+  u32 max_slots = 1;   // registers of the widest program
+  u32 n_compiled = 0;  // definitions with a compiled form
+};
+DeviceBook build_device_book(std::vector<hvm::DefRec> &defs, const std::vector<u64> &instrs);
+
 // Label sets of every definition (calc_labels.rs:78-194).
 std::map<std::string, LabSet> calculate_label_sets(const Book &book);
 

@@ -248,11 +248,14 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   const u32 wib = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // the book, copied into LDS when it fits (the host decides: P.book_lds)
   const u32 nd = P.ndefs * 4;
+  const u32 b_cdef = nd + P.instr_words + P.lab_words, b_tmpl = b_cdef + P.ndefs;
   if (P.book_lds) {
     const u64 *defs = (const u64 *)P.defs;
     for (u32 i = threadIdx.x; i < nd; i += blockDim.x) s_book[i] = defs[i];
     for (u32 i = threadIdx.x; i < P.instr_words; i += blockDim.x) s_book[nd + i] = P.instrs[i];
     for (u32 i = threadIdx.x; i < P.lab_words; i += blockDim.x) s_book[nd + P.instr_words + i] = P.labbits[i];
+    for (u32 i = threadIdx.x; i < P.ndefs; i += blockDim.x) s_book[b_cdef + i] = P.cdefs[i];
+    for (u32 i = threadIdx.x; i < P.tmpl_words; i += blockDim.x) s_book[b_tmpl + i] = P.tmpl[i];
   }
   __syncthreads();
   const u32 gwave = blockIdx.x * WAVES_PER_BLOCK + wib;
@@ -273,6 +276,8 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     L.book = HVM_TO_LDS(s_book);
     L.book_ins = nd;
     L.book_lab = nd + P.instr_words;
+    L.book_cdef = b_cdef;
+    L.book_tmpl = b_tmpl;
   }
   L.bag = bag;
   L.bag_top = top_c;

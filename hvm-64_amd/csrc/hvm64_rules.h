@@ -141,9 +141,10 @@ struct Lane {
   const u32 *bag_bot;
   // the book in LDS (nullptr: read it from global memory): definition
   // records at 0 (4 words each), instructions at book_ins, label bits at
-  // book_lab
+  // book_lab, compiled-definition records at book_cdef, templates at
+  // book_tmpl
   lds_u64 *book;
-  u32 book_ins, book_lab;
+  u32 book_ins, book_lab, book_cdef, book_tmpl;
   // counters
   u32 rw[RW_SLOTS];
 };
@@ -608,6 +609,7 @@ struct Interp {
   u64 *Tg;        // trg registers LDS_SLOTS .. (global, stride 64)
   u32 left;       // instructions left; 0: no definition running
   bool t_lds;     // every trg register is in LDS (else all in global memory)
+  u64 tbase;      // compiled definition: address of its first template slot
 };
 
 // Trg register s of this lane: in LDS when every definition of the book
@@ -723,6 +725,41 @@ DEV void exec_num(Lane &L, Interp &I, u32 kind, u32 b, u32 c, u64 w1, u64 port, 
   }
 }
 
+// A template word spec (hvm64_engine.h) as the word it stands for.
+DEV u64 tspec(const Interp &I, u64 s) {
+  const u32 t = ptag(s);
+  if (t == T_REF || t == T_NUM) return s;
+  if (t >= T_OP) return (s & ~ADDR_MASK) | (I.tbase + (paddr(s) >> 3) * TSLOT_STRIDE);
+  if (t == T_VAR) {
+    const u64 k = paddr(s) >> 3;
+    return var_port(I.tbase + (k >> 1) * TSLOT_STRIDE + (k & 1) * 8);
+  }
+  if (s == P_NONE || s == P_FREE) return s;
+  return norm_endpoint(t_get(I, (u32)(paddr(s) >> 3)));
+}
+
+// I_TEMPLATE: writes every slot of the running definition's template (one
+// 16-byte store each), pushes its redexes and adds the counts of the
+// instructions the compiler evaluated.  The slots were reserved when the
+// call began (interp_begin).
+DEV void run_template(Lane &L, Interp &I, u32 off) {
+  const Params &P = *L.P;
+  const bool lds = L.book != nullptr;
+  const u32 base = lds ? L.book_tmpl + off : off;
+#define HVM_TW(i) (lds ? (u64)L.book[base + (i)] : P.tmpl[base + (i)])
+  const u64 h0 = HVM_TW(0), h1 = HVM_TW(1);
+  const u32 S = (u32)(h0 & 0xFF), R = (u32)((h0 >> 16) & 0xFF);
+  for (u32 j = 0; j < S; j++)
+    st2_new(P, I.tbase + j * TSLOT_STRIDE, tspec(I, HVM_TW(2 + 2 * j)), tspec(I, HVM_TW(3 + 2 * j)));
+  if (S) L.dirty = true;
+  for (u32 r = 0; r < R; r++) redux(L, tspec(I, HVM_TW(2 + 2 * S + 2 * r)), tspec(I, HVM_TW(3 + 2 * S + 2 * r)));
+#undef HVM_TW
+  if (h1) {
+#pragma unroll
+    for (u32 k = 0; k < 8; k++) L.rw[k] += (u32)((h1 >> (8 * k)) & 0xFF);
+  }
+}
+
 // One instruction (instruction.rs:110-210).  The common kinds come first
 // and take short paths: a wave executes the union of its lanes' paths, one
 // instruction every four cycles.
@@ -746,12 +783,17 @@ DEV void exec_step(Lane &L, Interp &I, u64 w0, u64 w1, u32 &np) {
     t_set(I, (u32)(w1 & 0xFFFF), wire_port(v + 8));
     return;
   }
-  const u64 ta = t_get(I, a);
-  if (kind == I_LINK || kind == I_LINKCONST) {
-    push_link(L, np, ta, kind == I_LINK ? t_get(I, b) : w1);
+  if (kind == I_TEMPLATE) {
+    run_template(L, I, (u32)w1);
+    return;
+  }
+  const u64 t_a = t_get(I, a);
+  if (kind == I_LINK || kind == I_LINKCONST || kind == I_LINKT) {
+    push_link(L, np, t_a, kind == I_LINK ? t_get(I, b) : kind == I_LINKT ? tspec(I, w1) : w1);
     return;
   }
   // Ctr / Op / OpNum / Switch: the trg, followed if it is a wire
+  const u64 ta = (w0 & I_ASWIRE) ? as_wire(t_a) : t_a;
   const u64 port = ptag(ta) == T_RED ? resolve(L, ta) : ta;
   if (kind != I_CTR) {
     exec_num(L, I, kind, b, c, w1, port, np);
@@ -777,15 +819,38 @@ DEV void exec_step(Lane &L, Interp &I, u64 w0, u64 w1, u32 &np) {
 }
 
 // def.rs:217-289: start the definition whose record is `d`, with t0 = `t0`.
-DEV void interp_begin(Lane &L, Interp &I, const DefRec &d, u64 t0) {
+// The compiled-definition record of definition di (0: none, or compiled
+// definitions are off).
+DEV u64 load_cdef(const Lane &L, u32 di) {
+  if (!L.P->compiled) return 0;
+  return L.book ? (u64)L.book[L.book_cdef + di] : L.P->cdefs[di];
+}
+
+// A compiled definition runs when its template's slots fit, as one block, in
+// what is left of the first lap of the lane's allocation slice (consecutive
+// slice positions are TSLOT_STRIDE bytes apart); otherwise the definition's
+// own instructions run.
+DEV void interp_begin(Lane &L, Interp &I, const DefRec &d, u64 t0, u64 cdef) {
   const Params &P = *L.P;
   I.Tl = HVM_TO_LDS(L.T_lds);
   I.Tg = L.T_glb;
   I.t_lds = P.t_lds && L.T_lds;
-  I.ip = P.instrs + (u64)d.ioff * 2;
-  I.ipl = L.book_ins + d.ioff * 2;
-  I.left = d.ilen;
-  if (d.nslots > P.trg_slots) {  // the trg area is sized for the book
+  u32 ioff = d.ioff, ilen = d.ilen, nslots = d.nslots;
+  I.tbase = 0;
+  if (cdef) {
+    const u32 ts = cdef_tslots(cdef);
+    if (L.apos + ts <= P.wave_nodes / 64) {
+      I.tbase = HVM_HEAP_OFF + (L.sub_node0 + L.apos * 64) * 16;
+      L.apos += ts;
+      ioff = cdef_ioff(cdef);
+      ilen = cdef_ilen(cdef);
+      nslots = cdef_nslots(cdef);
+    }
+  }
+  I.ip = P.instrs + (u64)ioff * 2;
+  I.ipl = L.book_ins + ioff * 2;
+  I.left = ilen;
+  if (nslots > P.trg_slots) {  // the trg area is sized for the book
     fail(L, ERR_BADPTR, BAD_SLOTS);
     I.left = 0;
     return;
@@ -850,7 +915,7 @@ DEV void expand_call(Lane &L, Interp &I, u32 &np, u64 cell, u64 port) {
       return;
     }
     L.rw[RW_DREF]++;
-    interp_begin(L, I, load_def(L, def_index(oaddr)), expand_ref(cell));
+    interp_begin(L, I, load_def(L, def_index(oaddr)), expand_ref(cell), load_cdef(L, def_index(oaddr)));
     return;
   }
   u64 out = ld(P, cell);
@@ -896,7 +961,7 @@ DEV void call(Lane &L, Interp &I, u32 &np, u64 port, u64 trg) {
     return;
   }
   L.rw[RW_DREF]++;
-  interp_begin(L, I, d, trg);
+  interp_begin(L, I, d, trg, load_cdef(L, di));
 }
 
 // interact.rs:6-44.  The arms are tested in the reference's order; each arm
@@ -969,12 +1034,15 @@ DEV void run_task(Lane &L, bool has, u64 a, u64 b, u32 di, u64 t0) {
   I.t_lds = false;
   u32 np = 0;
   if (has) interact(L, I, np, a, b);
-  else if (di != NO_DEF) interp_begin(L, I, load_def(L, di), t0);
+  else if (di != NO_DEF) interp_begin(L, I, load_def(L, di), t0, load_cdef(L, di));
   for (;;) {
     if (I.left > 0) interp_run(L, I, np);
     if (np > 0) flush_links(L, np);
     if (I.left == 0) break;
   }
+  // nodes a template wrote may be reached only through the redexes it
+  // pushed: retire their stores before any lane reads them
+  publish_if_dirty(L);
 }
 
 // ---------------------------------------------------------------------------
@@ -998,6 +1066,8 @@ DEV void lane_init(Lane &L, const Params &P, u32 gwave, u32 lane) {
   L.book = nullptr;
   L.book_ins = 0;
   L.book_lab = 0;
+  L.book_cdef = 0;
+  L.book_tmpl = 0;
 #pragma unroll
   for (u32 k = 0; k < RW_SLOTS; k++) L.rw[k] = 0;
 }

@@ -33,7 +33,7 @@ EXPORTS = [
     "hvm_gpu_reduce", "hvm_gpu_rewrites", "hvm_gpu_shortcuts", "hvm_gpu_readback", "hvm_gpu_last_kernel_ms",
     "hvm_gpu_grid", "hvm_gpu_set_timeout", "hvm_gpu_sched_stats", "hvm_gpu_open_grid",
     "hvm_gpu_boot_batch", "hvm_gpu_normal_batch", "hvm_gpu_root_ports", "hvm_gpu_readback_root",
-    "hvm_gpu_snapshot", "hvm_gpu_load_book_image",
+    "hvm_gpu_snapshot", "hvm_gpu_load_book_image", "hvm_gpu_set_compiled",
     "hvm_book_print", "hvm_book_label_sets", "hvm_net_apply_tree", "hvm_book_encode",
 ]
 
@@ -108,6 +108,7 @@ def load_library(path=None):
     lib.hvm_gpu_snapshot.argtypes = [P, ctypes.POINTER(NetImage), u64, u64, ctypes.POINTER(u64)]
     lib.hvm_gpu_load_book.argtypes = [P, ctypes.POINTER(BookBlob)]
     lib.hvm_gpu_load_book_image.argtypes = [P, ctypes.c_char_p, u64]
+    lib.hvm_gpu_set_compiled.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]
     lib.hvm_book_encode.argtypes = [ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
     for name in ("hvm_book_print", "hvm_book_label_sets"):
         getattr(lib, name).argtypes = [ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
@@ -181,6 +182,13 @@ class Heap:
     def set_timeout(self, seconds):
         """Wall-clock limit of one reduction launch (HVM_E_TIMEOUT past it)."""
         self._check(self._lib.hvm_gpu_set_timeout(self._h, float(seconds)))
+
+    def set_compiled(self, on=True):
+        """Compiled definitions on or off (default on); returns how many
+        definitions of the loaded book have a compiled form."""
+        n = ctypes.c_uint32()
+        self._check(self._lib.hvm_gpu_set_compiled(self._h, 1 if on else 0, ctypes.byref(n)))
+        return n.value
 
     def grid(self):
         b, w = ctypes.c_uint32(), ctypes.c_uint32()

@@ -178,6 +178,16 @@ int hvm_gpu_snapshot(hvm_gpu *h, hvm_net_image *img, uint64_t first, uint64_t n,
  * kernel's stream), and the reduce grid: blocks and waves. */
 int hvm_gpu_last_kernel_ms(hvm_gpu *h, double *ms);
 
+/* Compiled definitions on (on != 0, the default) or off.  At book load each
+ * definition is also compiled into a residual program (its instructions that
+ * depend on the port meeting it, interpreted as InterpretedDef::call does,
+ * runtime/src/def.rs:217-289) and a template of everything else, evaluated
+ * on the host (the role of the reference's compiled definitions,
+ * src/compile.rs:110-161).  Normal forms and counts are the same either way.
+ * *n_compiled (if non-null) receives how many definitions of the loaded book
+ * have a compiled form. */
+int hvm_gpu_set_compiled(hvm_gpu *h, int on, uint32_t *n_compiled);
+
 /* Wall-clock limit of one reduction launch (default 600 s; 0 = none).  A
  * launch that runs past it stops every wave and returns HVM_E_TIMEOUT. */
 int hvm_gpu_set_timeout(hvm_gpu *h, double seconds);
