@@ -511,9 +511,13 @@ CompiledDef compile_def(const u64 *instr, u32 ninstr, u32 nslots) {
     for (u64 c = 0; c < C.cells.size(); c++)
       if (cell_slot[c] != ~0ull) slot_words[cell_slot[c]] = P_NONE;
     tmpl.insert(tmpl.end(), slot_words.begin(), slot_words.end());
-    for (auto &r : C.redexes) {
-      tmpl.push_back(spec(r.first));
-      tmpl.push_back(spec(r.second));
+    // Redexes are pushed last-made first, the order in which the
+    // interpreter's link list delivers them: the oldest redexes of a wave's
+    // rings are the ones handed to idle waves, and the definitions' last
+    // redexes are their recursive calls (the larger pieces of work).
+    for (auto it = C.redexes.rbegin(); it != C.redexes.rend(); ++it) {
+      tmpl.push_back(spec(it->first));
+      tmpl.push_back(spec(it->second));
     }
 
     // residual program

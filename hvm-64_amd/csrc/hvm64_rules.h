@@ -886,12 +886,6 @@ DEV void interp_run(Lane &L, Interp &I, u32 &np) {
       // path too.
       HVM_SETTLE_LOADS();
     }
-    // The links the residual instructions made are linked before the
-    // template is written: the redexes they create (for example a number
-    // meeting a duplicator the definition made) come before the calls the
-    // template pushes, as in the definition's own instruction order.
-    if ((u32)(w0 & 0xFF) == I_TEMPLATE && np > 0) break;
-    if (n > 1 && (u32)(w2 & 0xFF) == I_TEMPLATE) n = 1;
 #pragma clang loop unroll(disable)
     for (u32 j = 0; j < n; j++) exec_step(L, I, j ? w2 : w0, j ? w3 : w1, np);
     I.ip += 2 * n;
