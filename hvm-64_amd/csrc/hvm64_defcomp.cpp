@@ -28,6 +28,7 @@
 // remaining links.  Rewrite counts are those of the reference: every static
 // instruction counts exactly what its interpretation would.
 #include <algorithm>
+#include <tuple>
 #include <map>
 #include <stdexcept>
 #include <vector>
@@ -78,6 +79,13 @@ struct Compiler {
   std::vector<Cell> cells;
   std::vector<ANode> nodes;
   std::vector<std::pair<AV, AV>> redexes;
+  // Order keys of the redexes: the interpreter defers a definition's links
+  // to a list of LINK_CAP entries and links a full list the last four
+  // first, so its redexes appear in that order; the template pushes its
+  // redexes in the same order (seq: the link request that made the redex,
+  // counted in instruction order; depth: continuations after it).
+  std::vector<std::pair<u64, u64>> redex_seq;
+  u64 link_seq = 0, cur_seq = 0;
   std::vector<ROp> pattern;  // residual instructions (before the template)
   std::vector<ROp> links;    // residual links (after the template)
   u64 ntok = 1;              // token 0 = t0
@@ -122,7 +130,10 @@ struct Compiler {
     b = norm(b);
     if (principal(a) && principal(b)) {
       if (skippable(a) && skippable(b)) cnt[2]++;
-      else redexes.push_back({a, b});
+      else {
+        redexes.push_back({a, b});
+        redex_seq.push_back({cur_seq, (u64)depth});
+      }
       return;
     }
     if (a.k != AK::End && b.k == AK::End) std::swap(a, b);
@@ -148,6 +159,31 @@ struct Compiler {
     r.in = a.v;
     r.other = b;
     links.push_back(r);
+  }
+
+  // A link an instruction asks for (one entry of the interpreter's list).
+  void ilink(AV a, AV b) {
+    cur_seq = link_seq++;
+    link(a, b);
+  }
+
+  // The interpreter's order of the redexes: full lists of LINK_CAP links in
+  // turn; within a list the last four links first, then the rest (and the
+  // continuations of earlier rounds).
+  std::vector<size_t> redex_order() const {
+    const u64 cap = 6;
+    std::vector<std::tuple<u64, u64, u64, size_t>> keys;
+    for (size_t i = 0; i < redexes.size(); i++) {
+      const u64 s = redex_seq[i].first, d = redex_seq[i].second;
+      const u64 batch = s / cap, pos = s % cap;
+      const u64 m = std::min<u64>(cap, link_seq - batch * cap);
+      const u64 round = (pos + 4 >= m ? 0 : 1) + d;
+      keys.push_back({batch, round, pos, i});
+    }
+    std::sort(keys.begin(), keys.end());
+    std::vector<size_t> order;
+    for (auto &k : keys) order.push_back(std::get<3>(k));
+    return order;
   }
 
   // Trg::target of an operand: a principal port, an unresolved endpoint
@@ -198,8 +234,8 @@ struct Compiler {
       const u32 a = (u32)(w0 >> 16) & 0xFFFF, b = (u32)(w0 >> 32) & 0xFFFF, c = (u32)(w0 >> 48);
       switch (kind) {
         case I_CONST: T[a] = lit(w1); break;
-        case I_LINK: link(T[a], T[b]); break;
-        case I_LINKCONST: link(T[a], lit(w1)); break;
+        case I_LINK: ilink(T[a], T[b]); break;
+        case I_LINKCONST: ilink(T[a], lit(w1)); break;
         case I_WIRES: {
           const u64 c1 = new_cell(), c2 = new_cell();
           const u32 d = (u32)(w1 & 0xFFFF);
@@ -233,7 +269,7 @@ struct Compiler {
     }
     const u64 v1 = new_cell(), v2 = new_cell();
     const u64 m = new_node(T_CTR, lab, {AK::End, v1}, {AK::End, v2});
-    link(t, node_port(m));
+    ilink(t, node_port(m));
     T[b] = {AK::End, v1};
     T[c] = {AK::End, v2};
   }
@@ -257,7 +293,7 @@ struct Compiler {
     }
     const u64 v1 = new_cell(), v2 = new_cell();
     const u64 m = new_node(T_OP, opc, {AK::End, v1}, {AK::End, v2});
-    link(t, node_port(m));
+    ilink(t, node_port(m));
     T[b] = {AK::End, v1};
     T[c] = {AK::End, v2};
   }
@@ -282,7 +318,7 @@ struct Compiler {
     }
     const u64 v = new_cell();
     const u64 m = new_node(T_OP, opc, lit(numport(rhs)), {AK::End, v});
-    link(t, node_port(m));
+    ilink(t, node_port(m));
     T[c] = {AK::End, v};
   }
 
@@ -315,7 +351,7 @@ struct Compiler {
     }
     const u64 v1 = new_cell(), v2 = new_cell();
     const u64 m = new_node(T_SWI, 0, {AK::End, v1}, {AK::End, v2});
-    link(t, node_port(m));
+    ilink(t, node_port(m));
     T[b] = {AK::End, v1};
     T[c] = {AK::End, v2};
   }
@@ -328,6 +364,9 @@ struct Compiler {
   // ports (comm02, interact.rs:162-168).  The template then holds what is
   // left, and a call makes those interactions no more.
   void prereduce() {
+    // redexes made here come after every list
+    cur_seq = (link_seq / 6 + 1) * 6;
+    link_seq = cur_seq + 1;
     for (u64 guard = 0;; guard++) {
       if (guard > 1 << 16) throw Fail("prereduce");
       bool done = false;
@@ -342,6 +381,7 @@ struct Compiler {
           if (!same) continue;
           if (!na.alive || !nb.alive) throw Fail("redex of a taken node");
           redexes.erase(redexes.begin() + (long)i);
+          redex_seq.erase(redex_seq.begin() + (long)i);
           na.alive = nb.alive = false;
           cnt[0]++;
           link(na.w[0], nb.w[0]);
@@ -350,6 +390,7 @@ struct Compiler {
         } else if (a.k == AK::Lit && (a.v == P_ERA || (ptag(a.v) == T_NUM && nb.tag == T_CTR))) {
           if (!nb.alive) throw Fail("redex of a taken node");
           redexes.erase(redexes.begin() + (long)i);
+          redex_seq.erase(redex_seq.begin() + (long)i);
           nb.alive = false;
           cnt[1]++;
           link(nb.w[0], a);
@@ -511,13 +552,12 @@ CompiledDef compile_def(const u64 *instr, u32 ninstr, u32 nslots) {
     for (u64 c = 0; c < C.cells.size(); c++)
       if (cell_slot[c] != ~0ull) slot_words[cell_slot[c]] = P_NONE;
     tmpl.insert(tmpl.end(), slot_words.begin(), slot_words.end());
-    // Redexes are pushed last-made first, the order in which the
-    // interpreter's link list delivers them: the oldest redexes of a wave's
-    // rings are the ones handed to idle waves, and the definitions' last
-    // redexes are their recursive calls (the larger pieces of work).
-    for (auto it = C.redexes.rbegin(); it != C.redexes.rend(); ++it) {
-      tmpl.push_back(spec(it->first));
-      tmpl.push_back(spec(it->second));
+    // Redexes are pushed in the order the interpreter's link list would
+    // deliver them: which redexes are oldest in a wave's rings decides which
+    // ones are handed to idle waves.
+    for (size_t i : C.redex_order()) {
+      tmpl.push_back(spec(C.redexes[i].first));
+      tmpl.push_back(spec(C.redexes[i].second));
     }
 
     // residual program
