@@ -26,6 +26,8 @@ u32 reduce_waves_per_block();
 u32 reduce_book_lds_words();
 u32 reduce_lds_slots();
 hipError_t reduce_occupancy(int *blocks_per_cu);
+hipError_t noalloc_scratch(u64 **out);
+u32 noalloc_nets();
 }  // namespace hvm
 
 using namespace hvm;
@@ -54,6 +56,7 @@ struct hvm_gpu {
   u32 blocks = 0, waves = 0;
   u64 *d_rb_stack = nullptr, *d_rb_out = nullptr;
   u64 *d_batch = nullptr;  // batch staging: roots, def ids, ports
+  u64 *scratch = nullptr;  // this device's static staging array (hvm_gpu_boot_batch_noalloc)
   u64 batch_cap = 0;       // words
   u64 rb_cap = 0;
   hvm64::Host host;
@@ -375,6 +378,7 @@ int hvm_gpu_open_grid(int device, uint64_t heap_bytes, uint32_t max_blocks, hvm_
     hip_check(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device), "wall clock rate");
     h->wall_khz = wall_khz > 0 ? (u64)wall_khz : 100000;
     h->timeout_ticks = h->wall_khz * 1000 * 600;  // 10 minutes unless the caller sets one
+    hip_check(noalloc_scratch(&h->scratch), "staging array address");
     upload_book(h.get());
     reset_net(h.get(), true);
   } catch (Error &e) {
@@ -598,6 +602,28 @@ int hvm_gpu_boot_batch(hvm_gpu *h, uint32_t n, const uint32_t *def_ids, uint64_t
     u64 *buf = batch_buffer(h, (u64)n + (n + 1) / 2);
     u64 *d_roots = buf;
     u32 *d_defs = (u32 *)(buf + n);
+    hip_check(hipMemcpyAsync(d_defs, def_ids, (size_t)n * 4, hipMemcpyHostToDevice, h->stream), "upload batch");
+    Params P = make_params(h, ~0ull);
+    hip_check(launch_boot_batch(P, n, d_defs, d_roots, h->waves, h->stream), "launch boot batch");
+    hip_check(hipMemcpyAsync(roots, d_roots, (size_t)n * 8, hipMemcpyDeviceToHost, h->stream), "download roots");
+    sync_globals(h);
+  });
+}
+
+// hvm_gpu_boot_batch staged through the device's static array instead of
+// the handle's staging buffer: no device or host allocation.  Calls that
+// overlap on one GPU share that array and race.
+int hvm_gpu_boot_batch_noalloc(hvm_gpu *h, uint32_t n, const uint32_t *def_ids, uint64_t *roots) {
+  return guarded(h, [&] {
+    if (n == 0) return;
+    if (!def_ids || !roots) throw Error(HVM_E_ARG, "null batch array");
+    if (n > noalloc_nets())
+      throw Error(HVM_E_ARG, "batch of " + std::to_string(n) + " nets exceeds the allocation-free maximum of " +
+                                 std::to_string(noalloc_nets()));
+    for (u32 i = 0; i < n; i++)
+      if (def_ids[i] >= h->ndefs) throw Error(HVM_E_ARG, "no such definition in batch");
+    u64 *d_roots = h->scratch;
+    u32 *d_defs = (u32 *)(h->scratch + noalloc_nets());
     hip_check(hipMemcpyAsync(d_defs, def_ids, (size_t)n * 4, hipMemcpyHostToDevice, h->stream), "upload batch");
     Params P = make_params(h, ~0ull);
     hip_check(launch_boot_batch(P, n, d_defs, d_roots, h->waves, h->stream), "launch boot batch");

@@ -18,6 +18,9 @@
 // the cell and keeps linking.  There is no lock and no spin on another lane's
 // progress, so divergent lanes of one wave can never wait on each other.
 #include <hip/hip_runtime.h>
+
+#include <atomic>
+
 #include "hvm64_rules.h"
 
 namespace hvm {
@@ -711,6 +714,39 @@ __global__ void k_readback(Params P, u64 root_cell, u64 *stack, u64 stack_cap, u
   }
   P.g->result = n;
 }
+
+// ---------------------------------------------------------------------------
+// staging of the allocation-free batch boot (hvm_gpu_boot_batch_noalloc)
+// ---------------------------------------------------------------------------
+// NOALLOC_NETS root words, then NOALLOC_NETS 32-bit definition ids (in
+// NOALLOC_NETS / 2 words): 768 KiB, held by every GPU the library runs on
+// until the process ends.  Host code never passes `scratch` itself to a
+// kernel (on the host it is a stand-in, not a device address): it uses the
+// address hipGetSymbolAddress reports for the current device.
+static const u32 NOALLOC_NETS = 1u << 16;
+static __device__ __align__(256) u64 scratch[NOALLOC_NETS + NOALLOC_NETS / 2];
+static const int NOALLOC_MAX_DEVICES = 64;
+static std::atomic<u64 *> scratch_ptr[NOALLOC_MAX_DEVICES];
+
+// The device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy of the array at its own address).
+hipError_t noalloc_scratch(u64 **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+  u64 *p = scratch_ptr[dev].load();
+  if (!p) {
+    void *addr = nullptr;
+    e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    p = (u64 *)addr;
+    scratch_ptr[dev].store(p);
+  }
+  *out = p;
+  return hipSuccess;
+}
+u32 noalloc_nets() { return NOALLOC_NETS; }
 
 // ---------------------------------------------------------------------------
 // launch wrappers used by the host engine

@@ -33,6 +33,7 @@ EXPORTS = [
     "hvm_gpu_reduce", "hvm_gpu_rewrites", "hvm_gpu_shortcuts", "hvm_gpu_readback", "hvm_gpu_last_kernel_ms",
     "hvm_gpu_grid", "hvm_gpu_set_timeout", "hvm_gpu_sched_stats", "hvm_gpu_open_grid",
     "hvm_gpu_boot_batch", "hvm_gpu_normal_batch", "hvm_gpu_root_ports", "hvm_gpu_readback_root",
+    "hvm_gpu_boot_batch_noalloc",
     "hvm_gpu_snapshot", "hvm_gpu_load_book_image", "hvm_gpu_set_compiled",
     "hvm_book_print", "hvm_book_label_sets", "hvm_net_apply_tree", "hvm_book_encode",
 ]
@@ -102,6 +103,7 @@ def load_library(path=None):
     lib.hvm_gpu_sched_stats.argtypes = [P, ctypes.POINTER(u64)]
     lib.hvm_gpu_open_grid.argtypes = [ctypes.c_int, u64, ctypes.c_uint32, ctypes.POINTER(P)]
     lib.hvm_gpu_boot_batch.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(u64)]
+    lib.hvm_gpu_boot_batch_noalloc.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(u64)]
     lib.hvm_gpu_normal_batch.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(u64)]
     lib.hvm_gpu_root_ports.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(u64), ctypes.POINTER(u64)]
     lib.hvm_gpu_readback_root.argtypes = [P, u64, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
@@ -397,12 +399,15 @@ class Batch(Net):
         super().__init__(heap)
         self.roots = []
 
-    def boot(self, defs):
-        """Boots one net per entry of `defs` (Host.defs items or indices)."""
+    def boot(self, defs, allocation_free=False):
+        """Boots one net per entry of `defs` (Host.defs items or indices).
+        allocation_free: through hvm_gpu_boot_batch_noalloc (at most 65,536
+        nets, staged through a static device array)."""
         n = len(defs)
         ids = (ctypes.c_uint32 * n)(*[d.idx if hasattr(d, "idx") else int(d) for d in defs])
         roots = (ctypes.c_uint64 * n)()
-        self.heap._check(self.heap._lib.hvm_gpu_boot_batch(self.heap._h, n, ids, roots))
+        fn = self.heap._lib.hvm_gpu_boot_batch_noalloc if allocation_free else self.heap._lib.hvm_gpu_boot_batch
+        self.heap._check(fn(self.heap._h, n, ids, roots))
         self.roots = list(roots)
 
     def normal(self):

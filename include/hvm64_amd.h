@@ -144,6 +144,14 @@ int hvm_gpu_reduce(hvm_gpu *h, uint64_t limit, uint64_t *popped, int *is_normal)
  *   result; Var(root) while the root's wire is open).
  * hvm_gpu_readback_root: hvm_gpu_readback of one batch root. */
 int hvm_gpu_boot_batch(hvm_gpu *h, uint32_t n, const uint32_t *def_ids, uint64_t *roots);
+
+/* hvm_gpu_boot_batch for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops): the same boot, staged through a static
+ * 768 KiB device array instead of a buffer the handle allocates, and with no
+ * device or host allocation.  At most 65,536 nets per call (HVM_E_ARG
+ * beyond).  Every handle on one GPU shares that array: calls that overlap on
+ * one GPU race.  No counterpart in the reference. */
+int hvm_gpu_boot_batch_noalloc(hvm_gpu *h, uint32_t n, const uint32_t *def_ids, uint64_t *roots);
 int hvm_gpu_normal_batch(hvm_gpu *h, uint32_t n, uint64_t *roots);
 int hvm_gpu_root_ports(hvm_gpu *h, uint32_t n, const uint64_t *roots, uint64_t *ports);
 int hvm_gpu_readback_root(hvm_gpu *h, uint64_t root, char *buf, uint64_t cap, uint64_t *len);
