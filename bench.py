@@ -107,6 +107,23 @@ def cpu_baseline(name, size, seed, runs=5):
                      f"({single[0]['rwts']['rwts']} rewrites): {mips_1:.2f} MIPS, median of {runs} ({sec_1:.3f} s)",
            "single_thread_mips": round(mips_1, 2), "runs": runs, "nproc": nproc, "affinity": affinity,
            "cpu_share": CPU_SHARE, "cpu_model": cpu_model()}
+    # (iii) the compiled path: the book's definitions as C++ functions
+    # (src/compile.rs:110-161, do_switch for do_mat), same two runs
+    try:
+        import tempfile
+        work = tempfile.mkdtemp(prefix="hvm64_compiled_")
+        exe = o.compile_book(text, work)
+        nf_c, runs_t = o.run_compiled(exe, text, threads=threads, runs=runs, heap_words=1 << 29)
+        _, runs_1 = o.run_compiled(exe, small_text, threads=0, runs=runs, heap_words=1 << 28)
+        ok_c = nf_c == multi[0]["nf"] and runs_t[0][1] == multi[0]["rwts"]
+        out["compiled"] = {
+            "mips": round(_median([c["rwts"] / s / 1e6 for s, c in runs_t]), 2), "threads": threads,
+            "single_thread_mips": round(_median([c["rwts"] / s / 1e6 for s, c in runs_1]), 2),
+            "matches_interpreter": ok_c,
+            "sample": f"definitions compiled to C++ and built with g++ -O2; parallel_normal on size {size}, "
+                      f"normal() on size {small}, median of {runs} runs each"}
+    except Exception as e:  # g++ or the oracle sources missing: report it, keep the rest
+        out["compiled"] = {"mips": None, "sample": f"unavailable: {e}"}
     return out, multi[0]
 
 

@@ -740,6 +740,9 @@ struct alignas(32) Def {  // def.rs:86-95
   int kind = 0;  // 0 interpreted, 1 expand
   InterpretedDef data;
   Port out{0};  // ExpandDef payload (net.rs:72-74)
+  // A compiled definition (AsDef::call of the structs src/compile.rs:110-161
+  // writes): when set, calls run it instead of interpreting `data`.
+  void (*compiled)(Net &net, Port port) = nullptr;
 };
 static inline Port new_ref(const Def *d) { return Port::make(Ref, d->labs.min_safe, (uint64_t)d); }
 static inline Def *port_def(Port p) { return reinterpret_cast<Def *>(p.addr()); }
@@ -1076,6 +1079,7 @@ struct Net {
     if (trg.tag() == Ctr && !def->labs.has(trg.lab())) return comm02(port, trg);
     rwts.dref++;
     if (def->kind == 1) return expand_call(def, trg);
+    if (def->compiled) return def->compiled(*this, trg);
     interpret(def->data, Trg::port(trg));
   }
 
@@ -1778,6 +1782,145 @@ static RunResult run_book(const std::string &text, const std::string &entry, siz
   return res;
 }
 
+// ---------------------------------------------------------------------------
+// Compiled definitions (src/compile.rs:31-161): one C++ function per
+// definition, the instruction list written out as straight-line calls of
+// the interpreter's own do_* methods, as the reference's compiler writes
+// Rust (compile.rs:110-161), with do_switch where compile.rs:150 names a
+// do_mat that the runtime does not have.  The text is a translation unit
+// that includes this file and reduces a book with the compiled definitions
+// registered (main at the end): the CPU baseline of the compiled path.
+// ---------------------------------------------------------------------------
+static std::string compile_book_cpp(const std::string &text, const std::string &oracle_path) {
+  std::string out;
+  with_big_stack([&] {
+    Book book = parse_book(text);
+    Host host;
+    host.insert_book(book);
+    std::vector<std::string> names;
+    std::map<std::string, size_t> index;
+    for (auto &kv : host.defs) {
+      index[kv.first] = names.size();
+      names.push_back(kv.first);
+    }
+    auto port_text = [&](Port p) -> std::string {
+      char buf[64];
+      if (p.tag() == Ref && !p.is_era()) return "REF[" + std::to_string(index.at(host.back.at(p.addr()))) + "]";
+      snprintf(buf, sizeof buf, "Port{0x%llxull}", (unsigned long long)p.v);
+      return buf;
+    };
+    out += "// Generated: the definitions of a book compiled to C++ (src/compile.rs:110-161).\n";
+    out += "#include \"" + oracle_path + "\"\n";
+    out += "namespace oracle {\n";
+    out += "static Port REF[" + std::to_string(std::max<size_t>(names.size(), 1)) + "];\n";
+    for (size_t k = 0; k < names.size(); k++) {
+      const InterpretedDef &d = host.defs.at(names[k])->data;
+      out += "// @" + names[k] + "\n";
+      out += "static void def_" + std::to_string(k) + "(Net &net, Port port) {\n";
+      out += "  Trg t[" + std::to_string(std::max<uint32_t>(d.trgs, 1)) + "];\n";
+      out += "  t[0] = Trg::port(port);\n";
+      for (const Instruction &i : d.instr) {
+        std::string a = "t[" + std::to_string(i.a) + "]", b = "t[" + std::to_string(i.b) + "]";
+        std::string c = "t[" + std::to_string(i.c) + "]", dd = "t[" + std::to_string(i.d) + "]";
+        char buf[256];
+        switch (i.kind) {
+          case I_CONST: out += "  " + a + " = Trg::port(" + port_text(i.port) + ");\n"; break;
+          case I_LINK: out += "  net.link_trg(" + a + ", " + b + ");\n"; break;
+          case I_LINKCONST: out += "  net.link_trg_port(" + a + ", " + port_text(i.port) + ");\n"; break;
+          case I_CTR:
+            snprintf(buf, sizeof buf, "  { auto r = net.do_ctr(%u, %s); %s = r.first; %s = r.second; }\n", (unsigned)i.lab,
+                     a.c_str(), b.c_str(), c.c_str());
+            out += buf;
+            break;
+          case I_OP:
+            snprintf(buf, sizeof buf, "  { auto r = net.do_op((NumTag)%u, %s); %s = r.first; %s = r.second; }\n",
+                     (unsigned)i.lab, a.c_str(), b.c_str(), c.c_str());
+            out += buf;
+            break;
+          case I_OPNUM:
+            snprintf(buf, sizeof buf, "  %s = net.do_op_num((NumTag)%u, %s, Num{0x%xu});\n", c.c_str(), (unsigned)i.lab,
+                     a.c_str(), (unsigned)i.rhs.raw);
+            out += buf;
+            break;
+          case I_SWITCH:
+            out += "  { auto r = net.do_switch(" + a + "); " + b + " = r.first; " + c + " = r.second; }\n";
+            break;
+          case I_WIRES:
+            out += "  { uint64_t x = net.alloc(), y = other_half(x); " + a + " = Trg::port(Port::var(x)); " + b +
+                   " = Trg::wire(Wire{x}); " + c + " = Trg::port(Port::var(y)); " + dd + " = Trg::wire(Wire{y}); }\n";
+            break;
+        }
+      }
+      out += "}\n";
+    }
+    out += "static const char *NAMES[] = {";
+    for (auto &n : names) out += "\"" + n + "\", ";
+    out += "nullptr};\n";
+    out += "static void (*FNS[])(Net &, Port) = {";
+    for (size_t k = 0; k < names.size(); k++) out += "def_" + std::to_string(k) + ", ";
+    out += "nullptr};\n";
+    out += "static void register_compiled(Host &host) {\n";
+    out += "  for (size_t k = 0; NAMES[k]; k++) {\n";
+    out += "    Def *d = host.defs.at(NAMES[k]).get();\n";
+    out += "    d->compiled = FNS[k];\n";
+    out += "    REF[k] = new_ref(d);\n";
+    out += "  }\n";
+    out += "}\n";
+    out += "}  // namespace oracle\n";
+    out += "ORACLE_COMPILED_MAIN\n";
+  });
+  return out;
+}
+
+// The main of a compiled book: FILE [-t THREADS] [-e ENTRY] [-m WORDS]
+// [-r RUNS]; prints one line per run: seconds, the five counts, and (first
+// run) the normal form.  THREADS 0: normal(), else parallel_normal().
+#define ORACLE_COMPILED_MAIN                                                                        \
+  int main(int argc, char **argv) {                                                                \
+    using namespace oracle;                                                                        \
+    std::string file = argc > 1 ? argv[1] : "", entry = "main";                                    \
+    int threads = 0, runs = 1;                                                                     \
+    size_t words = (size_t)1 << 28;                                                                \
+    for (int k = 2; k + 1 < argc; k += 2) {                                                        \
+      std::string a = argv[k];                                                                     \
+      if (a == "-t") threads = atoi(argv[k + 1]);                                                  \
+      if (a == "-e") entry = argv[k + 1];                                                          \
+      if (a == "-m") words = strtoull(argv[k + 1], nullptr, 10);                                   \
+      if (a == "-r") runs = atoi(argv[k + 1]);                                                     \
+    }                                                                                              \
+    std::ifstream f(file);                                                                         \
+    std::stringstream ss;                                                                          \
+    ss << f.rdbuf();                                                                               \
+    int rc = 0;                                                                                    \
+    with_big_stack([&] {                                                                           \
+      try {                                                                                        \
+        Book book = parse_book(ss.str());                                                          \
+        Host host;                                                                                 \
+        host.insert_book(book);                                                                    \
+        register_compiled(host);                                                                   \
+        for (int run = 0; run < runs; run++) {                                                     \
+          Heap heap(words);                                                                        \
+          Net net(&heap);                                                                          \
+          net.boot(host.defs.at(entry).get());                                                     \
+          auto t0 = std::chrono::steady_clock::now();                                              \
+          if (threads <= 0) net.normal();                                                          \
+          else net.parallel_normal((size_t)threads);                                               \
+          auto t1 = std::chrono::steady_clock::now();                                              \
+          const Rewrites &r = net.rwts;                                                            \
+          printf("%.6f %llu %llu %llu %llu %llu\n", std::chrono::duration<double>(t1 - t0).count(), \
+                 (unsigned long long)r.anni, (unsigned long long)r.comm, (unsigned long long)r.eras,  \
+                 (unsigned long long)r.dref, (unsigned long long)r.oper);                           \
+          if (run == 0) printf("%s\n", print_net(host.readback(net)).c_str());                     \
+          std::free(heap.nodes);                                                                   \
+        }                                                                                          \
+      } catch (HvmError &e) {                                                                      \
+        fprintf(stderr, "error: %s\n", e.what());                                                  \
+        rc = 1;                                                                                    \
+      }                                                                                            \
+    });                                                                                            \
+    return rc;                                                                                     \
+  }
+
 // Reduce an expression net against a book, as `hvm64 reduce` does (main.rs:176-193).
 static RunResult run_expr(const std::string &book_text, const std::string &expr, size_t heap_words) {
   RunResult res;
@@ -1899,6 +2042,17 @@ int oracle_print_book(const char *book, char *out, uint64_t cap, uint64_t *len) 
       text += "@" + kv.first + " = " + print_net(kv.second);
     }
     return copy_out(text, out, cap, len);
+  } catch (HvmError &e) {
+    g_err = e.what();
+    return 100 + e.code;
+  }
+}
+
+// A book's definitions compiled to C++ (compile_book_cpp); `oracle_path` is
+// the path of this file for the generated #include.
+int oracle_compile_book(const char *book, const char *oracle_path, char *out, uint64_t cap, uint64_t *len) {
+  try {
+    return copy_out(compile_book_cpp(book, oracle_path), out, cap, len);
   } catch (HvmError &e) {
     g_err = e.what();
     return 100 + e.code;

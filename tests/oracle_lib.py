@@ -38,6 +38,7 @@ class Oracle:
         lib.oracle_num_display.argtypes = [ctypes.c_uint32, ctypes.c_char_p, u64]
         lib.oracle_parse_num.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32)]
         lib.oracle_create_var.argtypes = [u64, ctypes.c_char_p, u64]
+        lib.oracle_compile_book.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
 
     def _err(self, rc):
         raise OracleError(f"oracle error {rc}: {self.lib.oracle_last_error().decode()}")
@@ -127,3 +128,54 @@ class Oracle:
         buf = ctypes.create_string_buffer(64)
         self.lib.oracle_create_var(i, buf, 64)
         return buf.value.decode()
+
+    def compile_book(self, book, workdir, opt="-O2"):
+        """The book's definitions compiled to C++ (src/compile.rs:110-161,
+        with do_switch), built with g++ into `workdir`; returns the path of
+        the executable (FILE [-t THREADS] [-e ENTRY] [-m WORDS] [-r RUNS])."""
+        import subprocess
+        src_path = os.path.join(ROOT, "oracle", "hvm64_oracle.cpp")
+        text = None
+        cap = 1 << 22
+        while text is None:
+            buf = ctypes.create_string_buffer(cap)
+            n = ctypes.c_uint64()
+            rc = self.lib.oracle_compile_book(book.encode(), src_path.encode(), buf, cap, ctypes.byref(n))
+            if rc == 7 and n.value + 1 > cap:
+                cap = n.value + 1
+                continue
+            if rc:
+                self._err(rc)
+            text = buf.value.decode()
+        os.makedirs(workdir, exist_ok=True)
+        cpp = os.path.join(workdir, "compiled_book.cpp")
+        exe = os.path.join(workdir, "compiled_book")
+        with open(cpp, "w") as f:
+            f.write(text)
+        subprocess.run(["g++", opt, "-std=c++20", "-pthread", "-w", cpp, "-o", exe], check=True)
+        return exe
+
+    @staticmethod
+    def run_compiled(exe, book, threads=0, runs=1, entry="main", heap_words=1 << 28):
+        """Runs a compiled book; returns (normal form, [(seconds, counts)])."""
+        import subprocess
+        import tempfile
+        with tempfile.NamedTemporaryFile("w", suffix=".hvm", delete=False) as f:
+            f.write(book)
+            path = f.name
+        try:
+            r = subprocess.run([exe, path, "-t", str(threads), "-r", str(runs), "-e", entry, "-m", str(heap_words)],
+                               capture_output=True, text=True, check=True)
+        finally:
+            os.unlink(path)
+        lines = r.stdout.split("\n")
+        out, nf, k = [], None, 0
+        for _ in range(runs):
+            parts = lines[k].split()
+            c = [int(x) for x in parts[1:]]
+            out.append((float(parts[0]), dict(anni=c[0], comm=c[1], eras=c[2], dref=c[3], oper=c[4], rwts=sum(c))))
+            k += 1
+            if nf is None:
+                nf = lines[k]
+                k += 1
+        return nf, out

@@ -491,3 +491,45 @@ def test_reduce_limit_bounds_pops(gpu_heap, oracle, limit):
     ref = oracle.run(text, heap_words=1 << 24)
     assert host.readback(net) == ref["nf"]
     assert net.rwts.as_dict() == ref["rwts"]
+
+
+# ---------------------------------------------------------------------------
+# interpreted and compiled definitions give the same results
+# ---------------------------------------------------------------------------
+INTERP_CASES = ["examples::sort::bitonic::bitonic_sort_lam", "tests::programs::f24", "tests::programs::u24",
+                "examples::church_encoding::church", "tests::programs::list_put_got"]
+
+
+@pytest.mark.parametrize("name", [n for n in INTERP_CASES if n in SNAPS])
+def test_interpreted_definitions_match_snapshot(gpu_heap, name):
+    """The same programs with compiled definitions off: every definition is
+    interpreted instruction by instruction (def.rs:217-289)."""
+    import hvm64
+    snap = SNAPS[name]
+    host = hvm64.Host(snap["text"], gpu_heap)
+    assert gpu_heap.set_compiled(False) > 0
+    try:
+        net = hvm64.Net(gpu_heap)
+        net.boot(host.defs["main"])
+        net.normal()
+        assert host.readback(net) == snap["normal_form"]
+        assert net.rwts.as_dict() == snap["rwts"]
+    finally:
+        gpu_heap.set_compiled(True)
+
+
+@pytest.mark.parametrize("text,shortcuts", SHORTCUT_BOOKS[:4])
+def test_era_shortcuts_interpreted(gpu_heap, oracle, text, shortcuts):
+    import hvm64
+    host = hvm64.Host(text, gpu_heap)
+    gpu_heap.set_compiled(False)
+    try:
+        net = hvm64.Net(gpu_heap)
+        net.boot(host.defs["main"])
+        net.normal()
+        ref = oracle.run(text, heap_words=1 << 16)
+        assert host.readback(net) == ref["nf"]
+        assert net.rwts.as_dict() == ref["rwts"]
+        assert net.shortcuts == ref["shortcuts"] == shortcuts
+    finally:
+        gpu_heap.set_compiled(True)
