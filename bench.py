@@ -177,43 +177,83 @@ def algorithmic_bytes_per_rewrite(name, size, seed):
 
 def bench_batch(args):
     """Config 4: args.nets independent nets (tree-sum depth U[8,16] or bitonic
-    k U[6,12], seeded) reduced in one launch, against one launch sequence per
-    net (Net::new + boot + normal) on a sample of the same nets."""
+    k U[6,12], seeded), each allocating from an arena of its own and all
+    reduced by one replay of a HIP graph (reset, boot, expand, reduce, root
+    ports) per step.  Beside it: the same nets through the unplanned batch
+    calls (boot_batch + normal_batch, shared allocation rings), and one
+    launch sequence per net (Net::new + boot + normal) on a seeded random
+    sample of the nets."""
+    import random
+
     import hvm64
     from hvm64 import configs
     book, nets, table = configs.batch_mix(args.nets, args.seed)
     heap = hvm64.Heap(int(args.heap_gb * (1 << 30)))
-    heap.set_timeout(300)
+    heap.set_timeout(60)
     host = hvm64.Host(book, heap)
     n_compiled = heap.set_compiled(not args.interpreted)
     defs = [host.defs[table[v][0]] for v in nets]
     expected = [hvm64.u24_port(int(table[v][1])) for v in nets]
 
-    def batched():
-        b = hvm64.Batch(heap)
-        b.boot(defs)
-        b.normal()
-        return b
+    # arenas: first from each variant's allocation count alone, then from
+    # what the plan's own first run asked of each arena
+    def note(msg):
+        print(f"[bench batch {time.perf_counter() - t_plan:7.2f} s] {msg}", file=sys.stderr, flush=True)
+
+    t_plan = time.perf_counter()
+    sizes = hvm64.arena_sizes(heap, defs, margin=1.3)
+    note(f"variants measured: {len(set(d.idx for d in defs))} definitions, {sum(sizes) * 16 / 2**30:.1f} GiB")
+    plan = hvm64.PlannedBatch(heap, defs, sizes)
+    note(f"planned: {plan.info()}")
+    plan.run()
+    note(f"first run: {plan.last_kernel_ms():.1f} ms, {plan.info()['pool_nodes']} nodes taken from the pool")
+    used = plan.used()
+    sizes = [max(1024, int(u * 1.1)) for u in used]
+    plan = hvm64.PlannedBatch(heap, defs, sizes)
+    plan_s = time.perf_counter() - t_plan
+    note(f"replanned: {sum(sizes) * 16 / 2**30:.1f} GiB of arenas")
+    tune_s = 0.0
+    if not args.no_tune:
+        t_tune = time.perf_counter()
+        plan.tune()
+        tune_s = time.perf_counter() - t_tune
+        note(f"tuned in {tune_s:.1f} s")
+    info = plan.info()
+    note(f"plan: {info}")
 
     for _ in range(args.warmup):
-        batched()
+        plan.run()
+    ok = True
     t0 = time.perf_counter()
     total = 0
     kernel_ms = []
     for _ in range(args.steps):
-        b = batched()
-        total += b.rwts.total()
-        kernel_ms.append(b.last_kernel_ms())
+        ports = plan.run()
+        ok = ok and ports == expected
+        total += plan.rwts.total()
+        kernel_ms.append(plan.last_kernel_ms())
     elapsed = time.perf_counter() - t0
-    ok = b.ports() == expected
-    rw = b.rwts
+    note(f"timed {args.steps} steps: {elapsed:.2f} s")
+    rw = plan.rwts
+    pool_nodes = plan.info()["pool_nodes"]
+    arena_mib = sum(sizes) * 16 / (1 << 20)
     batched_mips = total / elapsed / 1e6
 
+    # the same nets through the unplanned calls (shared allocation rings)
+    t2 = time.perf_counter()
+    b = hvm64.Batch(heap)
+    b.boot(defs)
+    b.normal()
+    un_elapsed = time.perf_counter() - t2
+    un_ok = b.ports() == expected
+    unplanned_mips = b.rwts.total() / un_elapsed / 1e6
+    note(f"unplanned batch: {un_elapsed:.2f} s")
+
     sample = min(args.per_net_sample, len(nets))
+    picks = sorted(random.Random(args.seed + 1).sample(range(len(nets)), sample))
     t1 = time.perf_counter()
     per_total = 0
-    per_ok = True
-    for i in range(sample):
+    for i in picks:
         net = hvm64.Net(heap)
         net.boot(defs[i])
         net.normal()
@@ -234,21 +274,25 @@ def bench_batch(args):
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic (seeded mix of in-net generated trees and keys)",
-        "config": {"workload": f"{args.nets} independent nets in one launch: tree-sum depth U[8,16] or bitonic "
-                               f"2^k keys k U[6,12] (seed {args.seed})",
+        "config": {"workload": f"{args.nets} independent nets, one graph replay per step: tree-sum depth U[8,16] "
+                               f"or bitonic 2^k keys k U[6,12] (seed {args.seed})",
                    "nets": args.nets, "seed": args.seed, "heap_gb": args.heap_gb, "grid_blocks": blocks, "waves": waves,
                    "definitions": "interpreted" if args.interpreted else f"compiled ({n_compiled})"},
         "rewrites_per_step": rw.total(),
         "rewrites": rw.as_dict(),
         "kernel_ms": round(sum(kernel_ms) / len(kernel_ms), 3),
         "result_ok": ok,
-        "per_net": {"mips": round(per_net_mips, 2), "nets_timed": sample, "rewrites": per_total,
-                    "seconds": round(per_elapsed, 3)},
+        "plan": {"tuning": info["tuning"], "graph": info["graph"], "tuned": not args.no_tune,
+                 "tune_seconds": round(tune_s, 2), "plan_seconds": round(plan_s, 2),
+                 "arena_mib": round(arena_mib, 1), "pool_nodes": pool_nodes},
+        "unplanned": {"mips": round(unplanned_mips, 2), "result_ok": un_ok, "seconds": round(un_elapsed, 4)},
+        "per_net": {"mips": round(per_net_mips, 2), "nets_timed": sample, "sample": f"seeded random (seed {args.seed + 1})",
+                    "rewrites": per_total, "seconds": round(per_elapsed, 3)},
         "batched_over_per_net": round(batched_mips / per_net_mips, 2),
     }
     print(json.dumps(out), flush=True)
     heap.close()
-    return 0 if ok else 1
+    return 0 if ok and un_ok else 1
 
 
 def bench_serve(args):
@@ -312,10 +356,12 @@ def main():
     ap.add_argument("--max-batch", type=int, default=1024, help="serve: requests admitted per batch at most")
     ap.add_argument("--per-net-sample", type=int, default=256,
                     help="batch: nets timed with one launch sequence each (the per-net baseline)")
+    ap.add_argument("--no-tune", action="store_true", help="batch: keep the default launch tuning")
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--heap-gb", type=float, default=None,
-                    help="heap size (default 192 GiB of the MI355X's 288 GB; serve: split over the slots)")
+                    help="heap size (default 192 GiB of the MI355X's 288 GB, batch 224 GiB; serve: split over the "
+                         "slots)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--interpreted", action="store_true",
                     help="run every definition interpreted (compiled definitions off)")
@@ -327,7 +373,9 @@ def main():
         # a larger heap keeps each lane's allocation slice on its first lap
         # (no reads to find free slots): bitonic 2^16 runs at ~1640 MIPS with
         # 32 GiB and ~1890 with 192 GiB
-        args.heap_gb = 192.0 / args.slots if args.workload == "serve" else 192.0
+        # batch: 4,096 arenas hold ~181 GiB (every node a net allocates,
+        # plus the ends of lanes' chunks); the pool grows them
+        args.heap_gb = 192.0 / args.slots if args.workload == "serve" else 224.0 if args.workload == "batch" else 192.0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs one net per "
               f"device and this build has no multi-GPU path).", file=sys.stderr)

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -20,6 +21,7 @@ hipError_t launch_reduce(const Params &P, u32 blocks, hipStream_t st);
 hipError_t launch_boot_batch(const Params &P, u32 n, const u32 *defs, u64 *roots, u32 max_blocks, hipStream_t st);
 hipError_t launch_expand_batch(const Params &P, u32 n, u64 *roots, u32 max_blocks, hipStream_t st);
 hipError_t launch_root_ports(const Params &P, u32 n, const u64 *roots, u64 *out, hipStream_t st);
+hipError_t launch_batch_reset(const Params &P, u64 pool_start, u64 ov_start, hipStream_t st);
 hipError_t launch_readback(const Params &P, u64 root_cell, u64 *stack, u64 stack_cap, u64 *out, u64 out_cap,
                            hipStream_t st);
 u32 reduce_waves_per_block();
@@ -57,6 +59,23 @@ struct hvm_gpu {
   u64 *d_rb_stack = nullptr, *d_rb_out = nullptr;
   u64 *d_batch = nullptr;  // batch staging: roots, def ids, ports
   u64 *scratch = nullptr;  // this device's static staging array (hvm_gpu_boot_batch_noalloc)
+  // the planned batch (hvm_gpu_batch_plan): device arrays in one buffer
+  struct Plan {
+    u32 n = 0;
+    u64 *buf = nullptr;  // roots | ports | arena_word | arena_base0 | pool_next | ids | arena_len | gmap
+    u64 *roots = nullptr, *ports = nullptr, *arena_word = nullptr, *arena_base0 = nullptr;
+    u64 *pool_next = nullptr;  // [0]: next pool granule, [1]: next overflow node
+    u32 *ids = nullptr, *arena_len = nullptr, *gmap = nullptr;
+    u64 gmap_len = 0, pool_start = 0, pool_end = 0;  // granules
+    u64 ov_start = 0;                                 // node index: the overflow region
+    u32 gran_shift = 0, refill = 1;
+    std::vector<u64> base0;        // per net: first granule of its initial extent
+    std::vector<u32> len;          // per net: granules of its initial extent
+    std::vector<u64> arena_nodes;  // as planned, per net
+    hvm_batch_tuning tuning{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;  // null: the steps are launched one by one
+  } plan;
   u64 batch_cap = 0;       // words
   u64 rb_cap = 0;
   hvm64::Host host;
@@ -69,6 +88,7 @@ struct hvm_gpu {
   u32 idle_words = 0;
   Globals last{};
   double last_ms = 0;
+  bool heap_dirty = false;  // arenas left words the allocation rings would read as live
   u64 timeout_ticks = 0;  // wall-clock ticks a reduction may run (0 = unbounded)
   u64 wall_khz = 100000;
   std::string err;
@@ -77,6 +97,13 @@ struct hvm_gpu {
 namespace {
 
 thread_local std::string g_open_error;
+
+// A planned batch captures the book's and the handle's device addresses:
+// whatever replaces them drops the plan.
+void drop_plan(hvm_gpu *h);
+
+// Granule-map entry of a granule no net owns (the device's NO_NET).
+const u32 NO_NET_HOST = 0xFFFFFFFFu;
 
 // First word of a book image written by hvm_book_encode: "HVM64BK1".
 const u64 BOOK_IMAGE_MAGIC = 0x314B4234364D5648ull;
@@ -148,6 +175,8 @@ void sync_globals(hvm_gpu *h) {
                       : e == ERR_BADPTR ? "invalid heap or book address (engine bug)"
                                           : "device error";
     if (e == ERR_BADPTR) throw Error(code, std::string(msg) + ", check " + std::to_string(h->last.err_info));
+    if (e == ERR_OOM && h->last.err_info == OOM_POOL) throw Error(code, "OOM: the arena pool ran dry");
+    if (e == ERR_OOM && h->last.err_info == OOM_OVERFLOW) throw Error(code, "OOM: the arena overflow region ran dry");
     throw Error(code, msg);
   }
 }
@@ -184,6 +213,7 @@ void upload_book(hvm_gpu *h) {
   // The device book: the definitions' own instructions, then their compiled
   // programs and templates (hvm64_defcomp.cpp).
   hvm64::DeviceBook b = hvm64::build_device_book(h->defs_host, h->instrs_host);
+  drop_plan(h);
   dfree(h->d_defs);
   dfree(h->d_instrs);
   dfree(h->d_labbits);
@@ -213,6 +243,8 @@ void upload_book(hvm_gpu *h) {
 // later laps only reuse slots whose words read FREE, so words left by a
 // previous net are either overwritten or skipped, never read as live.
 void reset_net(hvm_gpu *h, bool fill) {
+  fill = fill || h->heap_dirty;
+  h->heap_dirty = false;
   if (fill) hip_check(launch_fill(h->heap, h->heap_words, P_FREE, h->stream), "fill heap");
   hip_check(launch_fill(h->queue, h->qslots * 2, 0, h->stream), "clear ring");
   Globals g0{};
@@ -313,9 +345,333 @@ int guarded(hvm_gpu *h, F fn) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// planned batches: per-net arenas, one graph launch per batch
+// ---------------------------------------------------------------------------
+void drop_plan(hvm_gpu *h) {
+  hvm_gpu::Plan &p = h->plan;
+  if (p.exec) (void)hipGraphExecDestroy(p.exec);
+  if (p.graph) (void)hipGraphDestroy(p.graph);
+  dfree(p.buf);
+  p = hvm_gpu::Plan{};
+}
+
+hvm_batch_tuning default_tuning(const hvm_gpu *h) {
+  hvm_batch_tuning t{};
+  t.blocks = h->blocks;
+  t.mbox_keep = 256;
+  t.fill_below = 32;
+  t.arena_chunk = 32;
+  return t;
+}
+
+Params plan_params(hvm_gpu *h) {
+  hvm_gpu::Plan &p = h->plan;
+  Params P = make_params(h, ~0ull);
+  P.fill = 1;
+  P.total_waves = p.tuning.blocks * reduce_waves_per_block();
+  P.mbox_keep = p.tuning.mbox_keep;
+  P.fill_below = p.tuning.fill_below;
+  P.gmap = p.gmap;
+  P.gmap_len = p.gmap_len;
+  P.gran_shift = p.gran_shift;
+  P.arena_word = p.arena_word;
+  P.arena_base0 = p.arena_base0;
+  P.arena_len = p.arena_len;
+  P.pool_next = p.pool_next;
+  P.pool_end = p.pool_end;
+  P.ov_next = p.pool_next + 1;
+  P.ov_end = p.gmap_len << p.gran_shift;
+  P.n_nets = p.n;
+  P.refill_granules = p.refill;
+  P.arena_chunk = p.tuning.arena_chunk;
+  return P;
+}
+
+// The steps of one planned batch, on the handle's stream: reset, boot every
+// net into its arena, expand the roots, reduce, read the root ports.
+void plan_steps(hvm_gpu *h) {
+  hvm_gpu::Plan &p = h->plan;
+  Params P = plan_params(h);
+  hip_check(launch_batch_reset(P, p.pool_start, p.ov_start, h->stream), "launch batch reset");
+  hip_check(launch_boot_batch(P, p.n, p.ids, p.roots, p.tuning.blocks * reduce_waves_per_block(), h->stream),
+            "launch boot batch");
+  hip_check(launch_expand_batch(P, p.n, p.roots, p.tuning.blocks * reduce_waves_per_block(), h->stream),
+            "launch expand batch");
+  hip_check(launch_reduce(P, p.tuning.blocks, h->stream), "launch reduce");
+  hip_check(launch_root_ports(P, p.n, p.roots, p.ports, h->stream), "launch root ports");
+}
+
+// Lays out the arenas of n nets (each net's node bound rounded up to whole
+// granules, one after another; the rest of the heap is the pool their
+// arenas grow from), uploads the plan and captures its steps into a graph.
+void make_plan(hvm_gpu *h, u32 n, const u32 *def_ids, const u64 *arena_nodes, const hvm_batch_tuning &tuning) {
+  drop_plan(h);
+  hvm_gpu::Plan &p = h->plan;
+  p.n = n;
+  p.tuning = tuning;
+  p.arena_nodes.assign(arena_nodes, arena_nodes + n);
+  const u64 heap_nodes = h->heap_words / 2;
+  // granules: the smallest power of two of at least 1,024 nodes that keeps
+  // the granule map within 4M entries
+  u32 gs = 10;
+  while ((heap_nodes >> gs) > (4u << 20)) gs++;
+  p.gran_shift = gs;
+  p.gmap_len = heap_nodes >> gs;  // whole granules only
+  if (p.tuning.arena_chunk > (1u << gs)) p.tuning.arena_chunk = 1u << gs;
+  // refill extents: half a net's initial arena, within [16, 64] granules
+  p.refill = 64;
+  std::vector<u64> base0(n);
+  std::vector<u32> len(n);
+  std::vector<u32> gmap(p.gmap_len, NO_NET_HOST);
+  // granule 0 holds the heap's fixed cells
+  u64 cur = (RESERVED_WORDS / 2 + (1ull << gs) - 1) >> gs;
+  for (u32 i = 0; i < n; i++) {
+    const u64 gr = (std::max<u64>(arena_nodes[i], 1) + (1ull << gs) - 1) >> gs;
+    if (gr > 0xFFFFFFFFull || cur + gr > p.gmap_len)
+      throw Error(HVM_E_OOM, "the arenas of the batch exceed the heap (" + std::to_string(i) + " of " +
+                                 std::to_string(n) + " nets placed)");
+    base0[i] = cur;
+    len[i] = (u32)gr;
+    for (u64 g = cur; g < cur + gr; g++) gmap[g] = i;
+    cur += gr;
+  }
+  p.pool_start = cur;
+  // the last sixteenth of the free granules (at least 64) is the overflow
+  // region, served node by node while an extent is being replaced
+  const u64 free_g = p.gmap_len - cur;
+  const u64 ov_g = std::max<u64>(free_g / 16, 64);
+  if (ov_g >= free_g) throw Error(HVM_E_OOM, "the arenas of the batch leave no pool in the heap");
+  p.pool_end = p.gmap_len - ov_g;
+  p.ov_start = p.pool_end << gs;
+  const u64 words = 2 * (u64)n + 2 * (u64)n + 2 + (n + 1) / 2 * 2 + (p.gmap_len + 1) / 2 + 8;
+  hip_check(hipMalloc(&p.buf, words * 8), "hipMalloc batch plan");
+  u64 *w = p.buf;
+  p.roots = w;
+  w += n;
+  p.ports = w;
+  w += n;
+  p.arena_word = w;
+  w += n;
+  p.arena_base0 = w;
+  w += n;
+  p.pool_next = w;
+  w += 2;
+  p.ids = (u32 *)w;
+  w += (n + 1) / 2;
+  p.arena_len = (u32 *)w;
+  w += (n + 1) / 2;
+  p.gmap = (u32 *)w;
+  std::vector<u64> word0(n);
+  for (u32 i = 0; i < n; i++) word0[i] = base0[i] << 32;
+  const u64 pool0[2] = {p.pool_start, p.ov_start};
+  hip_check(hipMemcpy(p.arena_word, word0.data(), (size_t)n * 8, hipMemcpyHostToDevice), "upload arenas");
+  hip_check(hipMemcpy(p.arena_base0, base0.data(), (size_t)n * 8, hipMemcpyHostToDevice), "upload arenas");
+  hip_check(hipMemcpy(p.pool_next, pool0, 16, hipMemcpyHostToDevice), "upload arenas");
+  hip_check(hipMemcpy(p.arena_len, len.data(), (size_t)n * 4, hipMemcpyHostToDevice), "upload arenas");
+  hip_check(hipMemcpy(p.ids, def_ids, (size_t)n * 4, hipMemcpyHostToDevice), "upload batch");
+  hip_check(hipMemcpy(p.gmap, gmap.data(), p.gmap_len * 4, hipMemcpyHostToDevice), "upload granule map");
+  p.base0 = std::move(base0);
+  p.len = std::move(len);
+  // the global redex ring starts empty (a completed reduction leaves it so)
+  hip_check(launch_fill(h->queue, h->qslots * 2, 0, h->stream), "clear ring");
+  hip_check(hipStreamSynchronize(h->stream), "clear ring");
+  // the arenas overwrite the heap: the handle's own net is gone
+  h->heap_dirty = true;
+  // capture the steps; without a graph they are launched one by one
+  hipGraph_t graph = nullptr;
+  if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+    bool ok = true;
+    try {
+      plan_steps(h);
+    } catch (Error &) {
+      ok = false;
+    }
+    if (hipStreamEndCapture(h->stream, &graph) != hipSuccess || !ok) {
+      if (graph) (void)hipGraphDestroy(graph);
+      graph = nullptr;
+    }
+  }
+  (void)hipGetLastError();
+  if (graph && hipGraphInstantiate(&p.exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+    p.graph = graph;
+  } else {
+    if (graph) (void)hipGraphDestroy(graph);
+    p.exec = nullptr;
+    (void)hipGetLastError();
+  }
+}
+
+// Per net, the nodes the last run took from its arena (every extent it
+// holds, less what is left of its current one), and the nodes the arenas
+// took from the pool and the overflow region.
+void plan_usage(hvm_gpu *h, std::vector<u64> &used, u64 &pool_nodes) {
+  hvm_gpu::Plan &p = h->plan;
+  std::vector<u64> word(p.n);
+  u64 next[2] = {0, 0};
+  hip_check(hipMemcpy(word.data(), p.arena_word, (size_t)p.n * 8, hipMemcpyDeviceToHost), "read arenas");
+  hip_check(hipMemcpy(next, p.pool_next, 16, hipMemcpyDeviceToHost), "read arenas");
+  const u64 pool_g = std::min(next[0], p.pool_end) - p.pool_start;
+  const u64 ov_end = p.gmap_len << p.gran_shift;
+  pool_nodes = (pool_g << p.gran_shift) + (std::min(next[1], ov_end) - p.ov_start);
+  std::vector<u32> gmap(pool_g);
+  if (pool_g) hip_check(hipMemcpy(gmap.data(), p.gmap + p.pool_start, pool_g * 4, hipMemcpyDeviceToHost), "read map");
+  std::vector<u64> extra(p.n, 0);  // pool granules per net
+  for (u32 net : gmap)
+    if (net < p.n) extra[net]++;
+  used.assign(p.n, 0);
+  const u64 G = 1ull << p.gran_shift;
+  for (u32 i = 0; i < p.n; i++) {
+    const u64 refill = std::min<u64>(std::max<u64>(p.len[i] >> 1, 16), p.refill);  // the device's refill_of
+    const u64 base = word[i] >> 32, off = word[i] & 0xFFFFFFFFull;
+    const u64 cur_len = (base == p.base0[i] ? p.len[i] : refill) * G;
+    const u64 total = (p.len[i] + extra[i]) * G;
+    used[i] = total - cur_len + std::min(off, cur_len);
+  }
+}
+
+// Plans the planned nets again (new tuning, or launch parameters changed).
+void replan(hvm_gpu *h, const hvm_batch_tuning t) {  // a copy: make_plan clears the plan first
+  hvm_gpu::Plan &p = h->plan;
+  if (!p.n) return;
+  const u32 n = p.n;
+  std::vector<u32> ids(n);
+  hip_check(hipMemcpy(ids.data(), p.ids, (size_t)n * 4, hipMemcpyDeviceToHost), "read batch");
+  const std::vector<u64> nodes = p.arena_nodes;
+  make_plan(h, n, ids.data(), nodes.data(), t);
+}
+
+// Runs the planned batch once; the duration of its steps goes to last_ms.
+void run_plan(hvm_gpu *h) {
+  hvm_gpu::Plan &p = h->plan;
+  if (!p.n) throw Error(HVM_E_ARG, "no batch planned");
+  hip_check(hipEventRecord(h->ev0, h->stream), "event");
+  if (p.exec) hip_check(hipGraphLaunch(p.exec, h->stream), "graph launch");
+  else plan_steps(h);
+  hip_check(hipEventRecord(h->ev1, h->stream), "event");
+  try {
+    sync_globals(h);
+  } catch (Error &) {
+    // a failed reduction may leave redexes in the global ring
+    (void)launch_fill(h->queue, h->qslots * 2, 0, h->stream);
+    (void)hipStreamSynchronize(h->stream);
+    throw;
+  }
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "elapsed");
+  h->last_ms = ms;
+}
+
 }  // namespace
 
 extern "C" {
+
+int hvm_gpu_batch_plan(hvm_gpu *h, uint32_t n, const uint32_t *def_ids, const uint64_t *arena_nodes,
+                       const hvm_batch_tuning *tuning) {
+  return guarded(h, [&] {
+    if (n == 0 || !def_ids || !arena_nodes) throw Error(HVM_E_ARG, "empty batch");
+    for (u32 i = 0; i < n; i++)
+      if (def_ids[i] >= h->ndefs) throw Error(HVM_E_ARG, "no such definition in batch");
+    hvm_batch_tuning t = tuning ? *tuning : default_tuning(h);
+    if (t.blocks == 0 || t.blocks > h->blocks) t.blocks = h->blocks;
+    if (t.arena_chunk < 4) t.arena_chunk = 4;
+    if (t.arena_chunk > 4096) t.arena_chunk = 4096;
+    make_plan(h, n, def_ids, arena_nodes, t);
+  });
+}
+
+int hvm_gpu_batch_run(hvm_gpu *h, uint64_t *ports, uint64_t *roots) {
+  return guarded(h, [&] {
+    run_plan(h);
+    hvm_gpu::Plan &p = h->plan;
+    if (ports) hip_check(hipMemcpy(ports, p.ports, (size_t)p.n * 8, hipMemcpyDeviceToHost), "download ports");
+    if (roots) hip_check(hipMemcpy(roots, p.roots, (size_t)p.n * 8, hipMemcpyDeviceToHost), "download roots");
+  });
+}
+
+int hvm_gpu_batch_info(hvm_gpu *h, hvm_batch_tuning *tuning, int *graph, uint64_t *pool_nodes) {
+  return guarded(h, [&] {
+    hvm_gpu::Plan &p = h->plan;
+    if (tuning) *tuning = p.tuning;
+    if (graph) *graph = p.exec != nullptr;
+    if (pool_nodes) {
+      *pool_nodes = 0;
+      if (p.n) {
+        std::vector<u64> used;
+        plan_usage(h, used, *pool_nodes);
+      }
+    }
+  });
+}
+
+int hvm_gpu_batch_used(hvm_gpu *h, uint64_t *used) {
+  return guarded(h, [&] {
+    hvm_gpu::Plan &p = h->plan;
+    if (!p.n) throw Error(HVM_E_ARG, "no batch planned");
+    if (!used) throw Error(HVM_E_ARG, "null argument");
+    std::vector<u64> u;
+    u64 pool = 0;
+    plan_usage(h, u, pool);
+    std::copy(u.begin(), u.end(), used);
+  });
+}
+
+int hvm_gpu_batch_tune(hvm_gpu *h, hvm_batch_tuning *best) {
+  return guarded(h, [&] {
+    hvm_gpu::Plan &p = h->plan;
+    if (!p.n) throw Error(HVM_E_ARG, "no batch planned");
+    hvm_batch_tuning cur = p.tuning;
+    const auto t0 = std::chrono::steady_clock::now();
+    double best_ms = 1e300;
+    // a candidate's first replay uploads its graph; the second is timed,
+    // unless the first was already far slower than the best so far
+    auto time_of = [&](const hvm_batch_tuning &t) {
+      replan(h, t);
+      run_plan(h);
+      if (h->last_ms > 4 * best_ms) return h->last_ms;
+      run_plan(h);
+      return h->last_ms;
+    };
+    best_ms = time_of(cur);
+    // one parameter at a time, the others fixed at their best so far; at
+    // most TUNE_SECONDS of trials
+    const double TUNE_SECONDS = 30;
+    const u32 full = h->blocks;
+    const std::vector<std::vector<u32>> cand = {
+        {full, full * 3 / 4, full / 2}, {64, 128, 256, 512}, {16, 32, 48}, {8, 16, 32}};
+    for (size_t k = 0; k < cand.size(); k++) {
+      for (u32 v : cand[k]) {
+        hvm_batch_tuning t = cur;
+        u32 *field = k == 0 ? &t.blocks : k == 1 ? &t.mbox_keep : k == 2 ? &t.fill_below : &t.arena_chunk;
+        if (*field == v || v == 0) continue;
+        if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > TUNE_SECONDS) break;
+        *field = v;
+        const double ms = time_of(t);
+        if (ms < best_ms) {
+          best_ms = ms;
+          cur = t;
+        }
+      }
+    }
+    replan(h, cur);
+    if (best) *best = cur;
+  });
+}
+
+int hvm_gpu_alloc_count(hvm_gpu *h, uint64_t *nodes) {
+  return guarded(h, [&] {
+    if (!nodes) throw Error(HVM_E_ARG, "null argument");
+    std::vector<u64> pos((size_t)h->waves * 64);
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    hip_check(hipMemcpy(pos.data(), h->alloc_pos, pos.size() * 8, hipMemcpyDeviceToHost), "read rings");
+    // ring positions advanced, less the slots found in use, plus the
+    // overflow ring's allocations
+    hip_check(hipMemcpy(&h->last, h->g, sizeof(Globals), hipMemcpyDeviceToHost), "read globals");
+    u64 sum = h->last.rw[RW_OV_ALLOC];
+    for (u64 v : pos) sum += v;
+    *nodes = sum - std::min(sum, (u64)h->last.rw[RW_ALLOC_MISS]);
+  });
+}
 
 int hvm_gpu_open(int device, uint64_t heap_bytes, hvm_gpu **out) {
   return hvm_gpu_open_grid(device, heap_bytes, 0, out);
@@ -394,6 +750,7 @@ void hvm_gpu_close(hvm_gpu *h) {
   if (!h) return;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  drop_plan(h);
   dfree(h->heap);
   dfree(h->alloc_pos);
   dfree(h->idle_bits);
@@ -750,6 +1107,7 @@ int hvm_gpu_set_compiled(hvm_gpu *h, int on, uint32_t *n_compiled) {
   return guarded(h, [&] {
     h->compiled = on != 0;
     if (n_compiled) *n_compiled = h->n_compiled;
+    replan(h, h->plan.tuning);
   });
 }
 
@@ -757,6 +1115,7 @@ int hvm_gpu_set_timeout(hvm_gpu *h, double seconds) {
   return guarded(h, [&] {
     if (seconds < 0) throw Error(HVM_E_ARG, "negative timeout");
     h->timeout_ticks = (u64)(seconds * (double)h->wall_khz * 1000.0);
+    replan(h, h->plan.tuning);
   });
 }
 

@@ -74,6 +74,9 @@ enum : u32 {
   RW_SLOTS
 };
 
+// Which arena store ran dry (Globals::err_info of ERR_OOM in a planned batch).
+enum : u32 { OOM_POOL = 1, OOM_OVERFLOW = 2 };
+
 // Which consistency check reported ERR_BADPTR (Globals::err_info).
 enum : u32 { BAD_HEAP_ADDR = 1, BAD_LINK_LIST, BAD_CHAIN, BAD_RESOLVE, BAD_SLOTS, BAD_DEF_CALL, BAD_DEF_EXPAND };
 
@@ -150,6 +153,30 @@ struct Params {
   u32 fill;            // busy waves short of work also claim from the global ring
   u64 limit;           // redex budget for reduce(limit); ~0 = unlimited
   u64 timeout_ticks;   // wall-clock (100 MHz) budget of a launch; 0 = none
+  // scheduler thresholds (tunable): redexes a busy wave keeps before handing
+  // some to an idle wave; pending redexes under which a busy wave of a
+  // batched launch claims from the global ring
+  u32 mbox_keep, fill_below;
+  // Per-net arenas (batched nets; gmap == nullptr: the per-lane rings).  The
+  // heap is cut into granules of 2^gran_shift nodes; gmap names each
+  // granule's net (or none).  Net i's arena starts as arena_len[i] granules
+  // from granule arena_base0[i] and grows by extents taken from the pool of
+  // granules [*pool_next, pool_end); arena_word[i] = current extent's base
+  // granule << 32 | nodes handed out of it.  Requests that arrive while an
+  // extent is being replaced take nodes [*ov_next, ov_end).
+  u32 *gmap;
+  u64 gmap_len;        // granules
+  u64 *arena_word;
+  const u64 *arena_base0;
+  const u32 *arena_len;
+  u64 *pool_next;      // next pool granule
+  u64 pool_end;
+  u64 *ov_next;        // next overflow node
+  u64 ov_end;
+  u32 n_nets;
+  u32 refill_granules; // largest refill extent (granules)
+  u32 arena_chunk;     // nodes a lane takes from its net's arena while it stays on the net
+  u32 gran_shift;
 };
 
 // Mailbox of a wave: a count word on its own 128-byte line, then up to

@@ -85,6 +85,8 @@ __global__ __launch_bounds__(64) void k_boot_batch(Params P, u32 n, const u32 *d
       roots[i] = 0;
       continue;
     }
+    L.cur_net = i;  // with arenas, net i allocates from arena i
+    L.net_known = true;
     u64 r = alloc_var1(L);
     roots[i] = r;
     L.rw[RW_DREF]++;
@@ -104,6 +106,8 @@ __global__ __launch_bounds__(64) void k_expand_batch(Params P, u32 n, u64 *roots
   L.PL = &s_pl[lane];
   for (u32 i = blockIdx.x * 64 + lane; i < n; i += gridDim.x * 64) {
     u64 old = roots[i];
+    L.cur_net = i;
+    L.net_known = true;
     u64 r2 = alloc_node(L), e = r2 + 8;
     st2(P, r2, P_NONE, var_port(r2));
     L.dirty = true;
@@ -115,6 +119,25 @@ __global__ __launch_bounds__(64) void k_expand_batch(Params P, u32 n, u64 *roots
   publish();
   lane_finish(L);
   wave_flush_counters(L, lane);
+}
+
+// Resets everything a batched launch reads before it starts, with stores
+// only (a HIP graph replays it ahead of every batch): the control block, the
+// idle set, the mailboxes, every net's arena (back to its initial extent),
+// the pool (from granule pool_start, owned by no net) and the overflow
+// region (from node ov_start).
+__global__ void k_batch_reset(Params P, u64 pool_start, u64 ov_start) {
+  const u64 tid = (u64)blockIdx.x * blockDim.x + threadIdx.x, stride = (u64)gridDim.x * blockDim.x;
+  u64 *gw = (u64 *)P.g;
+  for (u64 i = tid; i < sizeof(Globals) / 8; i += stride) gw[i] = 0;
+  for (u64 i = tid; i < P.idle_words; i += stride) P.idle_bits[i] = 0;
+  for (u64 i = tid; i < (u64)P.total_waves * MBOX_WORDS; i += stride) P.mbox[i] = 0;
+  for (u64 i = tid; i < P.n_nets; i += stride) P.arena_word[i] = P.arena_base0[i] << 32;
+  for (u64 i = pool_start + tid; i < P.pool_end; i += stride) P.gmap[i] = NO_NET;
+  if (tid == 0) {
+    *P.pool_next = pool_start;
+    *P.ov_next = ov_start;
+  }
 }
 
 // The port each root cell leads to (following parked ports without
@@ -178,6 +201,7 @@ static const u32 REFRESH = 64;        // busy iterations between control-word re
 static const u32 MAX_BACKOFF = 16384;  // idle backoff cap, in s_sleep(2) units (~0.9 ms)
 static const u32 MBOX_POLL = 16;       // s_sleep(2) units between mailbox polls (~1 us)
 static const u32 IDLE_CHECK = 1024;    // mailbox polls between reads of S (a power of two)
+// Defaults of the tunable thresholds (Params::mbox_keep, Params::fill_below).
 static const u32 MBOX_KEEP = 256;      // redexes a busy wave keeps before handing some off
 static const u32 FILL_BELOW = 32;      // batched nets: pending redexes under which a busy wave claims
 
@@ -288,6 +312,8 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   __builtin_amdgcn_wave_barrier();
   bool active = false;
   const bool limited = P.limit != ~0ull;
+  const u32 mbox_keep = P.mbox_keep ? P.mbox_keep : MBOX_KEEP;
+  const u32 fill_below = P.fill_below ? P.fill_below : FILL_BELOW;
   const u64 t_end = P.timeout_ticks ? wall_clock64() + P.timeout_ticks : ~0ull;
   u32 backoff = 1 + (gwave & 7);
   u32 busy = 0;                // busy iterations since the last control read
@@ -569,7 +595,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     // well stocked (the rule above did not fire), a wave holding more than
     // MBOX_KEEP redexes hands the excess to one of them; keeping more makes
     // fuller iterations (batched mix: 29% faster keeping 256 than 80)
-    if (use_mbox && idle_waves > 0 && n > MBOX_KEEP && spill == 0) spill = snc < n - MBOX_KEEP ? snc : n - MBOX_KEEP;
+    if (use_mbox && idle_waves > 0 && n > mbox_keep && spill == 0) spill = snc < n - mbox_keep ? snc : n - mbox_keep;
     const u32 spill0 = spill;
     if (spill && use_mbox && idle_waves > 0) {
       // hand up to MBOX_PAIRS of the excess to one idle wave
@@ -624,7 +650,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     // ---- batched nets: a busy wave short of work also claims from the
     // global ring, while few waves are idle.  With n < 64 pending, every
     // ring has room for the 64 - n it may take.
-    if (P.fill && !limited && spill0 == 0 && n < FILL_BELOW && (S_cache >> 32) > 0 &&
+    if (P.fill && !limited && spill0 == 0 && n < fill_below && (S_cache >> 32) > 0 &&
         idle_waves * 8 < P.total_waves) {
       u32 k = 0;
       u64 s_new = S_cache;
@@ -778,6 +804,10 @@ hipError_t launch_expand_batch(const Params &P, u32 n, u64 *roots, u32 max_block
   if (blocks > max_blocks) blocks = max_blocks;
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_expand_batch, dim3(blocks), dim3(64), 0, st, P, n, roots);
+  return hipGetLastError();
+}
+hipError_t launch_batch_reset(const Params &P, u64 pool_start, u64 ov_start, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_reset, dim3(1024), dim3(256), 0, st, P, pool_start, ov_start);
   return hipGetLastError();
 }
 hipError_t launch_root_ports(const Params &P, u32 n, const u64 *roots, u64 *out, hipStream_t st) {

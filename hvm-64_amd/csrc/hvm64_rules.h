@@ -127,6 +127,13 @@ struct Lane {
   u32 slice_rest;      // allocations left to take from the overflow ring
   u32 ov_left;         // nodes left in the current overflow run
   u64 ov_cur;          // next overflow-ring position of that run
+  // arenas: the net of the running task, and the lane's chunk of an arena
+  // (nodes [a_pos, a_end) of net a_net; a_net = NO_NET: none)
+  u32 cur_net, a_net;
+  u32 a_take;          // nodes of the lane's last take from a_net's arena
+  bool net_known;      // cur_net is the running task's (else: look it up from task_a, task_b)
+  u64 task_a, task_b;
+  u64 a_pos, a_end;
   bool dirty;          // stores of new nodes/cells not yet retired
   // links deferred by the running definition: pair i at PL[2i * 64],
   // PL[(2i + 1) * 64] (LDS; nullptr: link at once)
@@ -223,7 +230,120 @@ NOINL AllocCursor alloc_slow(u64 *heap, Globals *g, u64 wave_nodes, u64 ov_node0
   return c;
 }
 
+// Per-net arenas.  Net i's arena is a list of extents: its initial extent
+// (arena_len[i] granules from granule arena_base0[i]) and refill extents of
+// refill_of(i) granules taken from the shared pool.  One word per net,
+// arena_word[i] = base granule of the current extent << 32 | nodes handed
+// out of it, serves a lane's request with one atomic add.  The one lane
+// whose add crosses the extent's end takes the next extent from the pool,
+// names the net in the granule map for its granules and installs it; the
+// lanes whose adds land past the end before it is installed take their
+// nodes from the overflow region (node-granular, owned by no net).  No lane
+// waits for another.  Requests never exceed one granule (the host bounds the
+// chunk), so a fresh extent always holds the request that installed it.
+static const u32 NO_NET = 0xFFFFFFFFu;
+static const u32 ARENA_FIRST = 2;     // nodes taken right after the lane's net changed
+static const u32 ARENA_MIN_REFILL = 16;  // granules of the smallest refill extent
+
+struct ArenaChunk {
+  u64 pos, end;
+  u32 err, site;  // site: OOM_POOL or OOM_OVERFLOW
+};
+
+// Granules of net `net`'s refill extents: half its initial extent, within
+// [ARENA_MIN_REFILL, refill_granules] (an extent must outlast the window in
+// which it is replaced, or most requests would go to the overflow region).
+DEV u64 refill_of(const Params &P, u32 net) {
+  const u64 h = P.arena_len[net] >> 1;
+  return h < ARENA_MIN_REFILL ? ARENA_MIN_REFILL : h > P.refill_granules ? P.refill_granules : h;
+}
+
+NOINL ArenaChunk arena_take(const Params &P, u32 net, u32 n) {
+  ArenaChunk c;
+  c.err = 0;
+  c.site = 0;
+  u64 *word = &P.arena_word[net];
+  const u64 w = HVM_FAA(word, (u64)n);
+  const u64 base = w >> 32, off = w & 0xFFFFFFFFull;
+  const u64 len = (base == P.arena_base0[net] ? (u64)P.arena_len[net] : refill_of(P, net)) << P.gran_shift;
+  if (off + n <= len) {
+    c.pos = (base << P.gran_shift) + off;
+    c.end = c.pos + n;
+    return c;
+  }
+  if (off <= len) {  // this add crossed the end: the next extent is this lane's to install
+    const u64 r = refill_of(P, net);
+    const u64 g = HVM_FAA(P.pool_next, r);
+    if (g + r > P.pool_end) {
+      c.err = ERR_OOM;
+      c.site = OOM_POOL;
+      c.pos = c.end = 0;
+      return c;
+    }
+    for (u64 k = 0; k < r; k++) HVM_ST(&P.gmap[g + k], net);
+    HVM_ST(word, (g << 32) | n);
+    c.pos = g << P.gran_shift;
+    c.end = c.pos + n;
+    return c;
+  }
+  const u64 x = HVM_FAA(P.ov_next, (u64)n);
+  if (x + n > P.ov_end) {
+    c.err = ERR_OOM;
+    c.site = OOM_OVERFLOW;
+    c.pos = c.end = 0;
+    return c;
+  }
+  c.pos = x;
+  c.end = x + n;
+  return c;
+}
+
+// The net a redex belongs to: the arena of the node (or ExpandDef cell) it
+// reaches.  Two nilary ports allocate nothing, and a granule of the pool
+// whose owner this lane does not see yet names no net: the lane keeps its
+// net.
+DEV u32 net_of_port(const Params &P, u64 p, u32 dflt) {
+  const bool heap_ref = ptag(p) >= T_OP || (ptag(p) == T_REF && addr_in_heap(paddr(p)));
+  if (!heap_ref) return dflt;
+  const u64 g = ((paddr(p) - HVM_HEAP_OFF) >> 4) >> P.gran_shift;
+  if (g >= P.gmap_len) {
+    report_error(P.g, ERR_BADPTR, BAD_HEAP_ADDR);
+    return dflt;
+  }
+  const u32 net = HVM_LD(&P.gmap[g]);
+  return net < P.n_nets ? net : dflt;
+}
+
+// Makes room for `need` consecutive nodes of the running task's net.  Right
+// after the lane's net changed it takes little (a lane's next task is often
+// another net's, and the rest of a chunk would stay unused); while it stays
+// on one net each take doubles, up to the chunk.
+DEV void arena_room(Lane &L, u32 need) {
+  if (!L.net_known) {  // the task's net, looked up at its first allocation
+    L.cur_net = net_of_port(*L.P, L.task_a, net_of_port(*L.P, L.task_b, L.cur_net));
+    L.net_known = true;
+  }
+  const bool same = L.a_net == L.cur_net;
+  if (same && L.a_pos + need <= L.a_end) return;
+  const Params &P = *L.P;
+  u32 t = same ? L.a_take * 2 : ARENA_FIRST;
+  t = t > P.arena_chunk ? P.arena_chunk : t;
+  L.a_take = t;
+  const ArenaChunk c = arena_take(P, L.cur_net, need > t ? need : t);
+  L.a_net = L.cur_net;
+  L.a_pos = c.pos;
+  L.a_end = c.end;
+  if (c.err) {
+    fail(L, c.err, c.site);
+    L.a_end = L.a_pos;  // every later allocation reports again
+  }
+}
+
 DEV u64 alloc_node(Lane &L) {
+  if (L.P->gmap) {
+    arena_room(L, 1);
+    return L.a_pos < L.a_end ? HVM_HEAP_OFF + (L.a_pos++) * 16 : dump_addr();
+  }
   const u64 n = L.P->wave_nodes / 64;
   if (L.apos < n) return HVM_HEAP_OFF + (L.sub_node0 + (L.apos++) * 64) * 16;
   AllocCursor c;
@@ -506,6 +626,7 @@ DEV void comm22(Lane &L, u32 &np, u64 a, u64 b) {
   L.rw[RW_COMM]++;
   u64 a1, a2, b1, b2;
   take2(L, a, b, a1, a2, b1, b2);
+  if (L.P->gmap) arena_room(L, 6);  // the six nodes below, in one take
   u64 v1 = alloc_vars2(L), v3 = alloc_vars2(L);
   u64 v2 = v1 + 8, v4 = v3 + 8;
   u64 A1 = new_node(L, ptag(a), plab(a), var_port(v1), var_port(v2));
@@ -610,6 +731,7 @@ struct Interp {
   u32 left;       // instructions left; 0: no definition running
   bool t_lds;     // every trg register is in LDS (else all in global memory)
   u64 tbase;      // compiled definition: address of its first template slot
+  u64 tstride;    // bytes between its slots
 };
 
 // Trg register s of this lane: in LDS when every definition of the book
@@ -628,7 +750,7 @@ DEV void st2_new(const Params &P, u64 addr, u64 w0, u64 w1) {
 
 // Two fresh nodes: consecutive slice positions on the first lap.
 DEV void alloc2(Lane &L, u64 &n0, u64 &n1) {
-  if (L.apos + 2 <= L.P->wave_nodes / 64) {
+  if (!L.P->gmap && L.apos + 2 <= L.P->wave_nodes / 64) {
     n0 = HVM_HEAP_OFF + (L.sub_node0 + L.apos * 64) * 16;
     n1 = n0 + 64 * 16;
     L.apos += 2;
@@ -729,10 +851,10 @@ DEV void exec_num(Lane &L, Interp &I, u32 kind, u32 b, u32 c, u64 w1, u64 port, 
 DEV u64 tspec(const Interp &I, u64 s) {
   const u32 t = ptag(s);
   if (t == T_REF || t == T_NUM) return s;
-  if (t >= T_OP) return (s & ~ADDR_MASK) | (I.tbase + (paddr(s) >> 3) * TSLOT_STRIDE);
+  if (t >= T_OP) return (s & ~ADDR_MASK) | (I.tbase + (paddr(s) >> 3) * I.tstride);
   if (t == T_VAR) {
     const u64 k = paddr(s) >> 3;
-    return var_port(I.tbase + (k >> 1) * TSLOT_STRIDE + (k & 1) * 8);
+    return var_port(I.tbase + (k >> 1) * I.tstride + (k & 1) * 8);
   }
   if (s == P_NONE || s == P_FREE) return s;
   return norm_endpoint(t_get(I, (u32)(paddr(s) >> 3)));
@@ -750,7 +872,7 @@ DEV void run_template(Lane &L, Interp &I, u32 off) {
   const u64 h0 = HVM_TW(0), h1 = HVM_TW(1);
   const u32 S = (u32)(h0 & 0xFF), R = (u32)((h0 >> 16) & 0xFF);
   for (u32 j = 0; j < S; j++)
-    st2_new(P, I.tbase + j * TSLOT_STRIDE, tspec(I, HVM_TW(2 + 2 * j)), tspec(I, HVM_TW(3 + 2 * j)));
+    st2_new(P, I.tbase + j * I.tstride, tspec(I, HVM_TW(2 + 2 * j)), tspec(I, HVM_TW(3 + 2 * j)));
   if (S) L.dirty = true;
   for (u32 r = 0; r < R; r++) redux(L, tspec(I, HVM_TW(2 + 2 * S + 2 * r)), tspec(I, HVM_TW(3 + 2 * S + 2 * r)));
 #undef HVM_TW
@@ -828,7 +950,8 @@ DEV u64 load_cdef(const Lane &L, u32 di) {
 
 // A compiled definition runs when its template's slots fit, as one block, in
 // what is left of the first lap of the lane's allocation slice (consecutive
-// slice positions are TSLOT_STRIDE bytes apart); otherwise the definition's
+// slice positions are TSLOT_STRIDE bytes apart), or in the lane's chunk of
+// the net's arena (consecutive nodes); otherwise the definition's
 // own instructions run.
 DEV void interp_begin(Lane &L, Interp &I, const DefRec &d, u64 t0, u64 cdef) {
   const Params &P = *L.P;
@@ -837,11 +960,24 @@ DEV void interp_begin(Lane &L, Interp &I, const DefRec &d, u64 t0, u64 cdef) {
   I.t_lds = P.t_lds && L.T_lds;
   u32 ioff = d.ioff, ilen = d.ilen, nslots = d.nslots;
   I.tbase = 0;
+  I.tstride = TSLOT_STRIDE;
   if (cdef) {
     const u32 ts = cdef_tslots(cdef);
-    if (L.apos + ts <= P.wave_nodes / 64) {
+    bool fits = false;
+    if (P.gmap) {  // arenas: consecutive nodes of the lane's chunk
+      if (ts) arena_room(L, ts);
+      fits = L.a_pos + ts <= L.a_end;
+      if (fits) {
+        I.tbase = HVM_HEAP_OFF + L.a_pos * 16;
+        I.tstride = 16;
+        L.a_pos += ts;
+      }
+    } else if (L.apos + ts <= P.wave_nodes / 64) {
       I.tbase = HVM_HEAP_OFF + (L.sub_node0 + L.apos * 64) * 16;
       L.apos += ts;
+      fits = true;
+    }
+    if (fits) {
       ioff = cdef_ioff(cdef);
       ilen = cdef_ilen(cdef);
       nslots = cdef_nslots(cdef);
@@ -1033,6 +1169,11 @@ DEV void run_task(Lane &L, bool has, u64 a, u64 b, u32 di, u64 t0) {
   I.Tg = nullptr;
   I.t_lds = false;
   u32 np = 0;
+  if (has) {  // (with arenas, the net is looked up only if the task allocates)
+    L.task_a = a;
+    L.task_b = b;
+    L.net_known = false;
+  }
   if (has) interact(L, I, np, a, b);
   else if (di != NO_DEF) interp_begin(L, I, load_def(L, di), t0, load_cdef(L, di));
   for (;;) {
@@ -1057,6 +1198,13 @@ DEV void lane_init(Lane &L, const Params &P, u32 gwave, u32 lane) {
   L.ov_left = 0;
   L.ov_cur = 0;
   L.dirty = false;
+  L.cur_net = 0;
+  L.net_known = true;
+  L.task_a = L.task_b = 0;
+  L.a_net = NO_NET;
+  L.a_take = 0;
+  L.a_pos = 0;
+  L.a_end = 0;
   L.T_lds = nullptr;
   L.PL = nullptr;
   L.T_glb = P.trg_buf + (u64)gwave * P.trg_slots * 64 + lane;

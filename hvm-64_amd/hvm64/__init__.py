@@ -33,7 +33,8 @@ EXPORTS = [
     "hvm_gpu_reduce", "hvm_gpu_rewrites", "hvm_gpu_shortcuts", "hvm_gpu_readback", "hvm_gpu_last_kernel_ms",
     "hvm_gpu_grid", "hvm_gpu_set_timeout", "hvm_gpu_sched_stats", "hvm_gpu_open_grid",
     "hvm_gpu_boot_batch", "hvm_gpu_normal_batch", "hvm_gpu_root_ports", "hvm_gpu_readback_root",
-    "hvm_gpu_boot_batch_noalloc",
+    "hvm_gpu_boot_batch_noalloc", "hvm_gpu_batch_plan", "hvm_gpu_batch_run", "hvm_gpu_batch_tune",
+    "hvm_gpu_batch_info", "hvm_gpu_batch_used", "hvm_gpu_alloc_count",
     "hvm_gpu_snapshot", "hvm_gpu_load_book_image", "hvm_gpu_set_compiled",
     "hvm_book_print", "hvm_book_label_sets", "hvm_net_apply_tree", "hvm_book_encode",
 ]
@@ -61,6 +62,15 @@ class DefBlob(ctypes.Structure):
 class BookBlob(ctypes.Structure):
     """hvm_book_blob: an encoded book."""
     _fields_ = [("n_defs", ctypes.c_uint32), ("defs", ctypes.POINTER(DefBlob))]
+
+
+class BatchTuning(ctypes.Structure):
+    """hvm_batch_tuning: launch parameters of a planned batch."""
+    _fields_ = [("blocks", ctypes.c_uint32), ("mbox_keep", ctypes.c_uint32), ("fill_below", ctypes.c_uint32),
+                ("arena_chunk", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
 class HvmError(RuntimeError):
@@ -105,6 +115,14 @@ def load_library(path=None):
     lib.hvm_gpu_boot_batch.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(u64)]
     lib.hvm_gpu_boot_batch_noalloc.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(u64)]
     lib.hvm_gpu_normal_batch.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(u64)]
+    lib.hvm_gpu_batch_plan.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(u64),
+                                       ctypes.POINTER(BatchTuning)]
+    lib.hvm_gpu_batch_run.argtypes = [P, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+    lib.hvm_gpu_batch_tune.argtypes = [P, ctypes.POINTER(BatchTuning)]
+    lib.hvm_gpu_batch_info.argtypes = [P, ctypes.POINTER(BatchTuning), ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(u64)]
+    lib.hvm_gpu_alloc_count.argtypes = [P, ctypes.POINTER(u64)]
+    lib.hvm_gpu_batch_used.argtypes = [P, ctypes.POINTER(u64)]
     lib.hvm_gpu_root_ports.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(u64), ctypes.POINTER(u64)]
     lib.hvm_gpu_readback_root.argtypes = [P, u64, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
     lib.hvm_gpu_snapshot.argtypes = [P, ctypes.POINTER(NetImage), u64, u64, ctypes.POINTER(u64)]
@@ -423,6 +441,96 @@ class Batch(Net):
         out = (ctypes.c_uint64 * n)()
         self.heap._check(self.heap._lib.hvm_gpu_root_ports(self.heap._h, n, roots, out))
         return list(out)
+
+
+class PlannedBatch:
+    """A fixed list of nets (Net::new + Net::boot(defs[i]) + Net::normal()
+    each, runtime/src/net.rs:19-60) planned once and run again and again:
+    every net allocates from an arena of its own, and one run is one
+    replay of a HIP graph (reset, boot, expand, reduce, root ports)."""
+
+    def __init__(self, heap, defs, arena_nodes, tuning=None):
+        self.heap = heap
+        n = len(defs)
+        self.n = n
+        ids = (ctypes.c_uint32 * n)(*[d.idx if hasattr(d, "idx") else int(d) for d in defs])
+        nodes = (ctypes.c_uint64 * n)(*[int(x) for x in arena_nodes])
+        t = None
+        if tuning is not None:
+            t = BatchTuning(**tuning) if isinstance(tuning, dict) else tuning
+        heap._check(heap._lib.hvm_gpu_batch_plan(heap._h, n, ids, nodes, ctypes.byref(t) if t else None))
+        self.roots = []
+
+    def run(self):
+        """Runs the batch once; returns the port each net's root leads to."""
+        n = self.n
+        ports = (ctypes.c_uint64 * n)()
+        roots = (ctypes.c_uint64 * n)()
+        self.heap._check(self.heap._lib.hvm_gpu_batch_run(self.heap._h, ports, roots))
+        self.roots = list(roots)
+        return list(ports)
+
+    def used(self):
+        """Per net, the nodes the last run took from its arena (with the
+        extents it took from the pool)."""
+        out = (ctypes.c_uint64 * self.n)()
+        self.heap._check(self.heap._lib.hvm_gpu_batch_used(self.heap._h, out))
+        return list(out)
+
+    def tune(self):
+        best = BatchTuning()
+        self.heap._check(self.heap._lib.hvm_gpu_batch_tune(self.heap._h, ctypes.byref(best)))
+        return best.as_dict()
+
+    def info(self):
+        t = BatchTuning()
+        graph = ctypes.c_int()
+        pool = ctypes.c_uint64()
+        self.heap._check(self.heap._lib.hvm_gpu_batch_info(self.heap._h, ctypes.byref(t), ctypes.byref(graph),
+                                                           ctypes.byref(pool)))
+        return {"tuning": t.as_dict(), "graph": bool(graph.value), "pool_nodes": int(pool.value)}
+
+    @property
+    def rwts(self):
+        out = (ctypes.c_uint64 * 5)()
+        self.heap._check(self.heap._lib.hvm_gpu_rewrites(self.heap._h, out))
+        return Rewrites(*[int(x) for x in out])
+
+    @property
+    def shortcuts(self):
+        out = (ctypes.c_uint64 * 3)()
+        self.heap._check(self.heap._lib.hvm_gpu_shortcuts(self.heap._h, out))
+        return tuple(int(x) for x in out)
+
+    def last_kernel_ms(self):
+        ms = ctypes.c_double()
+        self.heap._check(self.heap._lib.hvm_gpu_last_kernel_ms(self.heap._h, ctypes.byref(ms)))
+        return ms.value
+
+
+def alloc_count(heap):
+    """Nodes the allocation rings handed out since the last reset."""
+    out = ctypes.c_uint64()
+    heap._check(heap._lib.hvm_gpu_alloc_count(heap._h, ctypes.byref(out)))
+    return int(out.value)
+
+
+def arena_sizes(heap, defs, margin=1.25, extra=1024):
+    """Arena sizes (nodes) for a planned batch of `defs`: each distinct
+    definition is reduced once alone with the allocation rings and its
+    allocation count, times `margin`, plus `extra`, sizes the arenas of its
+    nets.  The handle's own net is reset afterwards."""
+    need = {}
+    for d in defs:
+        idx = d.idx if hasattr(d, "idx") else int(d)
+        if idx in need:
+            continue
+        net = Net(heap)
+        net.boot(DefRef("", idx))
+        net.normal()
+        need[idx] = int(alloc_count(heap) * margin) + extra
+    Net(heap)
+    return [need[d.idx if hasattr(d, "idx") else int(d)] for d in defs]
 
 
 def u24_port(value):

@@ -156,6 +156,55 @@ int hvm_gpu_normal_batch(hvm_gpu *h, uint32_t n, uint64_t *roots);
 int hvm_gpu_root_ports(hvm_gpu *h, uint32_t n, const uint64_t *roots, uint64_t *ports);
 int hvm_gpu_readback_root(hvm_gpu *h, uint64_t root, char *buf, uint64_t cap, uint64_t *len);
 
+/* Planned batches: Net::boot + Net::normal per net (net.rs:30-32, 53-58)
+ * for a fixed list of nets, run again and again.
+ *
+ * hvm_gpu_batch_plan: plans n nets, net i booted with definition def_ids[i]
+ *   and allocating from an arena of its own: arena_nodes[i] 16-byte nodes
+ *   rounded up to whole granules (16 KiB on heaps up to 64 GiB, a larger
+ *   power of two above, so that at most 4M granules cover the heap), laid
+ *   out one after another.  The rest of the heap is a pool: a net that
+ *   outgrows its arena extends it from the pool by extents of half its
+ *   arena (16 to 64 granules); requests that arrive while an extent is
+ *   being installed are served from an overflow region (the pool's last
+ *   sixteenth).  HVM_E_OOM when either runs dry, or at plan time when the
+ *   arenas do not fit in the heap.  The run's steps (reset, boot, expand, reduce, root
+ *   ports) are captured in a HIP graph and replayed as one launch; where
+ *   capture is refused they are launched one by one.  The handle's own net
+ *   is gone after a plan (hvm_gpu_reset makes a fresh one).  tuning may be
+ *   null (defaults).
+ * hvm_gpu_batch_run: runs the plan once; ports[i] (may be null) receives the
+ *   port net i's root leads to, roots[i] (may be null) its root cell, for
+ *   hvm_gpu_readback_root.  hvm_gpu_rewrites then reports the batch's sums.
+ * hvm_gpu_batch_tune: times the plan under candidate tunings, one parameter
+ *   at a time (coordinate descent over grid size, mailbox and refill
+ *   thresholds and arena chunk), and keeps the fastest; *best receives it.
+ *   Trials stop after about 30 s.
+ * hvm_gpu_batch_info: the plan's tuning, whether it runs as a graph, and the
+ *   nodes the last run's arenas took from the pool and the overflow region.
+ * hvm_gpu_batch_used: per net, the nodes the last run took from its arena
+ *   and the extents it took from the pool (lanes take nodes in chunks, so
+ *   this includes the unused ends of chunks): the measure that sizes the
+ *   next plan's arenas.
+ * hvm_gpu_alloc_count: nodes allocated since the last reset by the per-lane
+ *   allocation rings (a single net or an unplanned batch): the measure that
+ *   sizes arenas.
+ * No counterpart in the reference (whose nets each own a heap). */
+typedef struct {
+  uint32_t blocks;       /* reduce workgroups (0: the handle's grid) */
+  uint32_t mbox_keep;    /* redexes a busy wave keeps before handing some to an idle wave */
+  uint32_t fill_below;   /* pending redexes under which a busy wave claims from the global ring */
+  uint32_t arena_chunk;  /* nodes a lane takes from its net's arena at a time while it stays on
+                            that net (at most one granule) */
+} hvm_batch_tuning;
+int hvm_gpu_batch_plan(hvm_gpu *h, uint32_t n, const uint32_t *def_ids, const uint64_t *arena_nodes,
+                       const hvm_batch_tuning *tuning);
+int hvm_gpu_batch_run(hvm_gpu *h, uint64_t *ports, uint64_t *roots);
+int hvm_gpu_batch_tune(hvm_gpu *h, hvm_batch_tuning *best);
+int hvm_gpu_batch_info(hvm_gpu *h, hvm_batch_tuning *tuning, int *graph, uint64_t *pool_nodes);
+int hvm_gpu_batch_used(hvm_gpu *h, uint64_t *used);
+int hvm_gpu_alloc_count(hvm_gpu *h, uint64_t *nodes);
+
 /* ~ Net.rwts (runtime/src/runtime.rs:72-79): anni, comm, eras, dref, oper. */
 int hvm_gpu_rewrites(hvm_gpu *h, uint64_t out[5]);
 
