@@ -125,3 +125,36 @@ def test_planned_batch_refuses_arenas_beyond_heap(gpu_heap):
     with pytest.raises(hvm64.HvmError) as e:
         hvm64.PlannedBatch(gpu_heap, defs, [1 << 40] * len(defs))
     assert e.value.kind == "OOM"
+
+
+def test_planned_batch_4096_against_oracle(oracle):
+    """The batched configuration at full size as a planned batch: 4,096 nets
+    in arenas of their own, sized from a first run's use, replayed twice;
+    every result is its variant's, and the five counts equal the sum of the
+    oracle's per-variant counts on both replays."""
+    import hvm64
+    from hvm64 import configs
+    print("case planned batch 4096 seed 7", flush=True)
+    book, nets, table = configs.batch_mix(4096, seed=7)
+    ref = {}
+    for v in table:
+        r = oracle.run(book, entry=table[v][0], heap_words=1 << 27)
+        assert r["nf"] == table[v][1]
+        ref[v] = r["rwts"]
+    total = {k: sum(ref[v][k] for v in nets) for k in ["rwts", "anni", "comm", "eras", "dref", "oper"]}
+    expected = [hvm64.u24_port(int(table[v][1])) for v in nets]
+    heap = hvm64.Heap(200 << 30)
+    try:
+        heap.set_timeout(60)
+        host = hvm64.Host(book, heap)
+        defs = [host.defs[table[v][0]] for v in nets]
+        plan = hvm64.PlannedBatch(heap, defs, hvm64.arena_sizes(heap, defs, margin=1.3))
+        plan.run()
+        plan = hvm64.PlannedBatch(heap, defs, [int(u * 1.1) + 1024 for u in plan.used()])
+        for rep in range(2):
+            ports = plan.run()
+            print("replay", rep, f"{plan.last_kernel_ms():.1f} ms", plan.info(), flush=True)
+            assert ports == expected
+            assert plan.rwts.as_dict() == total
+    finally:
+        heap.close()
