@@ -86,7 +86,6 @@ __global__ __launch_bounds__(64) void k_boot_batch(Params P, u32 n, const u32 *d
       continue;
     }
     L.cur_net = i;  // with arenas, net i allocates from arena i
-    L.net_known = true;
     u64 r = alloc_var1(L);
     roots[i] = r;
     L.rw[RW_DREF]++;
@@ -107,7 +106,6 @@ __global__ __launch_bounds__(64) void k_expand_batch(Params P, u32 n, u64 *roots
   for (u32 i = blockIdx.x * 64 + lane; i < n; i += gridDim.x * 64) {
     u64 old = roots[i];
     L.cur_net = i;
-    L.net_known = true;
     u64 r2 = alloc_node(L), e = r2 + 8;
     st2(P, r2, P_NONE, var_port(r2));
     L.dirty = true;
@@ -266,6 +264,9 @@ DEV u64 spill_to_global(const Params &P, u64 *bag, u32 *bot_c, u32 c, u32 cnt, u
   return __shfl(old, 0, 64) + ((u64)cnt << 32);
 }
 
+// ARENAS: compiled for per-net arenas (P.gmap set) or for the per-lane
+// allocation rings; each instantiation carries only its mode's allocator.
+template <bool ARENAS>
 __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   __shared__ u64 s_bag[WAVES_PER_BLOCK][RING_PAIRS * 2];
   __shared__ u64 s_T[WAVES_PER_BLOCK][(LDS_SLOTS + 2 * LINK_CAP) * 64];
@@ -297,6 +298,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   __builtin_amdgcn_wave_barrier();
   Lane L;
   lane_init(L, P, gwave, lane);
+  L.arenas = ARENAS;
   L.T_lds = &s_T[wib][lane];
   L.PL = &s_T[wib][LDS_SLOTS * 64 + lane];
   if (P.book_lds) {
@@ -818,7 +820,8 @@ hipError_t launch_root_ports(const Params &P, u32 n, const u64 *roots, u64 *out,
   return hipGetLastError();
 }
 hipError_t launch_reduce(const Params &P, u32 blocks, hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce, dim3(blocks), dim3(WAVES_PER_BLOCK * 64), 0, st, P);
+  if (P.gmap) hipLaunchKernelGGL(k_reduce<true>, dim3(blocks), dim3(WAVES_PER_BLOCK * 64), 0, st, P);
+  else hipLaunchKernelGGL(k_reduce<false>, dim3(blocks), dim3(WAVES_PER_BLOCK * 64), 0, st, P);
   return hipGetLastError();
 }
 hipError_t launch_readback(const Params &P, u64 root_cell, u64 *stack, u64 stack_cap, u64 *out, u64 out_cap,
@@ -830,7 +833,7 @@ u32 reduce_waves_per_block() { return WAVES_PER_BLOCK; }
 u32 reduce_book_lds_words() { return BOOK_LDS_WORDS; }
 u32 reduce_lds_slots() { return LDS_SLOTS; }
 hipError_t reduce_occupancy(int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_reduce, WAVES_PER_BLOCK * 64, 0);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_reduce<false>, WAVES_PER_BLOCK * 64, 0);
 }
 
 }  // namespace hvm

@@ -129,10 +129,12 @@ struct Lane {
   u64 ov_cur;          // next overflow-ring position of that run
   // arenas: the net of the running task, and the lane's chunk of an arena
   // (nodes [a_pos, a_end) of net a_net; a_net = NO_NET: none)
+  // (arenas: allocation from per-net arenas; a kernel that knows its mode
+  // sets it to a constant after lane_init, and the compiler drops the other
+  // mode's code from the inlined rules)
+  bool arenas;
   u32 cur_net, a_net;
   u32 a_take;          // nodes of the lane's last take from a_net's arena
-  bool net_known;      // cur_net is the running task's (else: look it up from task_a, task_b)
-  u64 task_a, task_b;
   u64 a_pos, a_end;
   bool dirty;          // stores of new nodes/cells not yet retired
   // links deferred by the running definition: pair i at PL[2i * 64],
@@ -250,44 +252,47 @@ struct ArenaChunk {
   u32 err, site;  // site: OOM_POOL or OOM_OVERFLOW
 };
 
-// Granules of net `net`'s refill extents: half its initial extent, within
-// [ARENA_MIN_REFILL, refill_granules] (an extent must outlast the window in
-// which it is replaced, or most requests would go to the overflow region).
-DEV u64 refill_of(const Params &P, u32 net) {
-  const u64 h = P.arena_len[net] >> 1;
-  return h < ARENA_MIN_REFILL ? ARENA_MIN_REFILL : h > P.refill_granules ? P.refill_granules : h;
+// Granules of a net's refill extents: half its initial extent (len
+// granules), within [ARENA_MIN_REFILL, max_refill] (an extent must outlast
+// the window in which it is replaced, or most requests would go to the
+// overflow region).
+DEV u64 refill_of(u64 len, u64 max_refill) {
+  const u64 h = len >> 1;
+  return h < ARENA_MIN_REFILL ? ARENA_MIN_REFILL : h > max_refill ? max_refill : h;
 }
 
-NOINL ArenaChunk arena_take(const Params &P, u32 net, u32 n) {
+// Out of line, with plain arguments: a reference to the kernel's Params
+// would make the compiler copy them to scratch memory.
+NOINL ArenaChunk arena_take(u64 *word, u64 base0, u64 len0, u64 max_refill, u32 gs, u64 *pool_next, u64 pool_end,
+                            u32 *gmap, u32 net, u64 *ov_next, u64 ov_end, u32 n) {
   ArenaChunk c;
   c.err = 0;
   c.site = 0;
-  u64 *word = &P.arena_word[net];
+  const u64 refill = refill_of(len0, max_refill);
   const u64 w = HVM_FAA(word, (u64)n);
   const u64 base = w >> 32, off = w & 0xFFFFFFFFull;
-  const u64 len = (base == P.arena_base0[net] ? (u64)P.arena_len[net] : refill_of(P, net)) << P.gran_shift;
+  const u64 len = (base == base0 ? len0 : refill) << gs;
   if (off + n <= len) {
-    c.pos = (base << P.gran_shift) + off;
+    c.pos = (base << gs) + off;
     c.end = c.pos + n;
     return c;
   }
   if (off <= len) {  // this add crossed the end: the next extent is this lane's to install
-    const u64 r = refill_of(P, net);
-    const u64 g = HVM_FAA(P.pool_next, r);
-    if (g + r > P.pool_end) {
+    const u64 g = HVM_FAA(pool_next, refill);
+    if (g + refill > pool_end) {
       c.err = ERR_OOM;
       c.site = OOM_POOL;
       c.pos = c.end = 0;
       return c;
     }
-    for (u64 k = 0; k < r; k++) HVM_ST(&P.gmap[g + k], net);
+    for (u64 k = 0; k < refill; k++) HVM_ST(&gmap[g + k], net);
     HVM_ST(word, (g << 32) | n);
-    c.pos = g << P.gran_shift;
+    c.pos = g << gs;
     c.end = c.pos + n;
     return c;
   }
-  const u64 x = HVM_FAA(P.ov_next, (u64)n);
-  if (x + n > P.ov_end) {
+  const u64 x = HVM_FAA(ov_next, (u64)n);
+  if (x + n > ov_end) {
     c.err = ERR_OOM;
     c.site = OOM_OVERFLOW;
     c.pos = c.end = 0;
@@ -319,17 +324,16 @@ DEV u32 net_of_port(const Params &P, u64 p, u32 dflt) {
 // another net's, and the rest of a chunk would stay unused); while it stays
 // on one net each take doubles, up to the chunk.
 DEV void arena_room(Lane &L, u32 need) {
-  if (!L.net_known) {  // the task's net, looked up at its first allocation
-    L.cur_net = net_of_port(*L.P, L.task_a, net_of_port(*L.P, L.task_b, L.cur_net));
-    L.net_known = true;
-  }
   const bool same = L.a_net == L.cur_net;
   if (same && L.a_pos + need <= L.a_end) return;
   const Params &P = *L.P;
   u32 t = same ? L.a_take * 2 : ARENA_FIRST;
   t = t > P.arena_chunk ? P.arena_chunk : t;
   L.a_take = t;
-  const ArenaChunk c = arena_take(P, L.cur_net, need > t ? need : t);
+  const u32 net = L.cur_net;
+  const ArenaChunk c = arena_take(&P.arena_word[net], P.arena_base0[net], P.arena_len[net], P.refill_granules,
+                                  P.gran_shift, P.pool_next, P.pool_end, P.gmap, net, P.ov_next, P.ov_end,
+                                  need > t ? need : t);
   L.a_net = L.cur_net;
   L.a_pos = c.pos;
   L.a_end = c.end;
@@ -340,7 +344,7 @@ DEV void arena_room(Lane &L, u32 need) {
 }
 
 DEV u64 alloc_node(Lane &L) {
-  if (L.P->gmap) {
+  if (L.arenas) {
     arena_room(L, 1);
     return L.a_pos < L.a_end ? HVM_HEAP_OFF + (L.a_pos++) * 16 : dump_addr();
   }
@@ -626,7 +630,7 @@ DEV void comm22(Lane &L, u32 &np, u64 a, u64 b) {
   L.rw[RW_COMM]++;
   u64 a1, a2, b1, b2;
   take2(L, a, b, a1, a2, b1, b2);
-  if (L.P->gmap) arena_room(L, 6);  // the six nodes below, in one take
+  if (L.arenas) arena_room(L, 6);  // the six nodes below, in one take
   u64 v1 = alloc_vars2(L), v3 = alloc_vars2(L);
   u64 v2 = v1 + 8, v4 = v3 + 8;
   u64 A1 = new_node(L, ptag(a), plab(a), var_port(v1), var_port(v2));
@@ -750,7 +754,7 @@ DEV void st2_new(const Params &P, u64 addr, u64 w0, u64 w1) {
 
 // Two fresh nodes: consecutive slice positions on the first lap.
 DEV void alloc2(Lane &L, u64 &n0, u64 &n1) {
-  if (!L.P->gmap && L.apos + 2 <= L.P->wave_nodes / 64) {
+  if (!L.arenas && L.apos + 2 <= L.P->wave_nodes / 64) {
     n0 = HVM_HEAP_OFF + (L.sub_node0 + L.apos * 64) * 16;
     n1 = n0 + 64 * 16;
     L.apos += 2;
@@ -964,7 +968,7 @@ DEV void interp_begin(Lane &L, Interp &I, const DefRec &d, u64 t0, u64 cdef) {
   if (cdef) {
     const u32 ts = cdef_tslots(cdef);
     bool fits = false;
-    if (P.gmap) {  // arenas: consecutive nodes of the lane's chunk
+    if (L.arenas) {  // arenas: consecutive nodes of the lane's chunk
       if (ts) arena_room(L, ts);
       fits = L.a_pos + ts <= L.a_end;
       if (fits) {
@@ -1169,11 +1173,7 @@ DEV void run_task(Lane &L, bool has, u64 a, u64 b, u32 di, u64 t0) {
   I.Tg = nullptr;
   I.t_lds = false;
   u32 np = 0;
-  if (has) {  // (with arenas, the net is looked up only if the task allocates)
-    L.task_a = a;
-    L.task_b = b;
-    L.net_known = false;
-  }
+  if (has && L.arenas) L.cur_net = net_of_port(*L.P, a, net_of_port(*L.P, b, L.cur_net));
   if (has) interact(L, I, np, a, b);
   else if (di != NO_DEF) interp_begin(L, I, load_def(L, di), t0, load_cdef(L, di));
   for (;;) {
@@ -1198,9 +1198,8 @@ DEV void lane_init(Lane &L, const Params &P, u32 gwave, u32 lane) {
   L.ov_left = 0;
   L.ov_cur = 0;
   L.dirty = false;
+  L.arenas = P.gmap != nullptr;
   L.cur_net = 0;
-  L.net_known = true;
-  L.task_a = L.task_b = 0;
   L.a_net = NO_NET;
   L.a_take = 0;
   L.a_pos = 0;
