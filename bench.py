@@ -261,6 +261,8 @@ def bench_batch(args):
     per_elapsed = time.perf_counter() - t1
     per_net_mips = per_total / per_elapsed / 1e6
     blocks, waves = heap.grid()
+    k_ms = sum(kernel_ms) / len(kernel_ms)
+    roofline = batch_roofline(book, table, nets, rw.total(), k_ms, args)
     out = {
         "metric": "MIPS on 1 MI355X: batched vs per-net launches",
         "value": round(batched_mips, 2),
@@ -280,8 +282,9 @@ def bench_batch(args):
                    "definitions": "interpreted" if args.interpreted else f"compiled ({n_compiled})"},
         "rewrites_per_step": rw.total(),
         "rewrites": rw.as_dict(),
-        "kernel_ms": round(sum(kernel_ms) / len(kernel_ms), 3),
+        "kernel_ms": round(k_ms, 3),
         "result_ok": ok,
+        "roofline": roofline,
         "plan": {"tuning": info["tuning"], "graph": info["graph"], "tuned": not args.no_tune,
                  "tune_seconds": round(tune_s, 2), "plan_seconds": round(plan_s, 2),
                  "arena_mib": round(arena_mib, 1), "pool_nodes": pool_nodes},
@@ -290,9 +293,65 @@ def bench_batch(args):
                     "rewrites": per_total, "seconds": round(per_elapsed, 3)},
         "batched_over_per_net": round(batched_mips / per_net_mips, 2),
     }
+    if not args.no_cpu:
+        out["cpu_baseline"] = batch_cpu_baseline(book, table, nets, args.seed)
     print(json.dumps(out), flush=True)
     heap.close()
     return 0 if ok and un_ok else 1
+
+
+def batch_roofline(book, table, nets, rewrites, k_ms, args):
+    """Roofline of one replay's k_reduce: the algorithmic heap traffic of
+    every net (the metered oracle on each variant: 8 B per load or store,
+    16 B per atomic RMW) over the replay's kernel time."""
+    try:
+        import oracle_lib
+        o = oracle_lib.Oracle()
+        per = {}
+        for v in table:
+            r = o.run(book, entry=table[v][0], heap_words=1 << 27, meter=True)
+            m = r["meter"]
+            per[v] = 8 * m["loads"] + 8 * m["stores"] + 16 * m["rmws"]
+    except Exception:
+        return None
+    alg = sum(per[v] for v in nets)
+    achieved = alg / (k_ms / 1e3) / 1e9
+    traffic, source = recorded_traffic("batch", args.nets, args.seed)
+    out = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": traffic,
+           "bytes_per_rewrite": round(alg / rewrites, 2), "kernel": "k_reduce<true>"}
+    if source:
+        out["traffic_source"] = source
+    return out
+
+
+def batch_cpu_baseline(book, table, nets, seed, sample=256):
+    """The oracle on a seeded random sample of the batch's nets, each net
+    reduced single-threaded (normal(), runtime/src/net.rs:55-60) and the nets
+    spread over the host cores (one net per worker at a time): the CPU's way
+    to batch independent nets."""
+    import random
+    from concurrent.futures import ThreadPoolExecutor
+    try:
+        import oracle_lib
+        o = oracle_lib.Oracle()
+    except Exception as e:
+        return {"value": None, "unit": "MIPS", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
+    try:
+        cores = max(1, min(len(os.sched_getaffinity(0)), CPU_SHARE))
+    except AttributeError:
+        cores = max(1, min(os.cpu_count() or 1, CPU_SHARE))
+    picks = random.Random(seed + 2).sample(range(len(nets)), min(sample, len(nets)))
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:  # ctypes calls release the GIL
+        runs = list(ex.map(lambda i: o.run(book, entry=table[nets[i]][0], heap_words=1 << 27, threads=0), picks))
+    sec = time.perf_counter() - t0
+    ok = all(r["nf"] == table[nets[i]][1] for r, i in zip(runs, picks))
+    total = sum(r["rwts"]["rwts"] for r in runs)
+    return {"value": round(total / sec / 1e6, 2), "unit": "MIPS", "cores": cores, "kind": "port",
+            "sample": f"{len(picks)} nets of the batch (seeded random, seed {seed + 2}), {total} rewrites, "
+                      f"single-threaded normal() per net over {cores} workers: {sec:.2f} s",
+            "results_ok": ok, "cpu_model": cpu_model()}
 
 
 def bench_serve(args):
@@ -382,6 +441,8 @@ def main():
         return 2
 
     import hvm64
+    if args.copy_mib:
+        copy_calibration(args.copy_mib << 20)
     if args.workload == "batch":
         return bench_batch(args)
     if args.workload == "serve":
@@ -400,9 +461,6 @@ def main():
         net.boot(main_def)
         net.normal()
         return net
-
-    if args.copy_mib:
-        copy_calibration(args.copy_mib << 20)
 
     for _ in range(args.warmup):
         net = step()
