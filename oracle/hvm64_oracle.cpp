@@ -11,8 +11,12 @@
 //                                               f24 "f32 op then RNE" closed forms
 //                                               (tests/test_oracle_num.py) plus the
 //                                               u24/i24/f24/numeric_casts snapshots.
-//                                               F24 `&` `|` `^` (atan2/log/powf):
-//                                               parity unpinned (no reference vector).
+//                                               F24 `&` `|` `^` (atan2/log/powf): no
+//                                               reference vector; pinned by exact
+//                                               identities and by seeded operands whose
+//                                               exact result lies far from an f24
+//                                               rounding boundary
+//                                               (tests/f24_transcendental.py).
 //   ast/parser ast/src/{ast,parser}.rs       -- every snapshot program parses and
 //                                               prints back to the snapshot text.
 //   host       host/src/*.rs                 -- calc_labels unit vector
@@ -215,7 +219,8 @@ static Num operate_sym(Num a, Num b) {  // num.rs:164-254
       case Ne: return Num::u24(p != q);
       case Lt: return Num::u24(p < q);
       case Gt: return Num::u24(p > q);
-      // parity unpinned: Rust std atan2/log/powf
+      // Rust std atan2 / log / powf in f32 (num.rs:243-248); pinned by
+      // identities and margin-checked operands, not by reference vectors
       case And: return Num::f24(atan2f(p, q));
       case Or: return Num::f24(logf(q) / logf(p));
       case Xor: return Num::f24(powf(p, q));
