@@ -22,8 +22,8 @@ hipError_t launch_boot_batch(const Params &P, u32 n, const u32 *defs, u64 *roots
 hipError_t launch_expand_batch(const Params &P, u32 n, u64 *roots, u32 max_blocks, hipStream_t st);
 hipError_t launch_root_ports(const Params &P, u32 n, const u64 *roots, u64 *out, hipStream_t st);
 hipError_t launch_batch_reset(const Params &P, u64 pool_start, u64 ov_start, hipStream_t st);
-hipError_t launch_readback(const Params &P, u64 root_cell, u64 *stack, u64 stack_cap, u64 *out, u64 out_cap,
-                           hipStream_t st);
+hipError_t launch_readback(const Params &P, u64 root_cell, u64 q_head, u64 q_tail, u64 *stack, u64 stack_cap,
+                           u64 *out, u64 out_cap, hipStream_t st);
 u32 reduce_waves_per_block();
 u32 reduce_book_lds_words();
 u32 reduce_lds_slots();
@@ -280,7 +280,19 @@ void run_reduce(hvm_gpu *h, u64 limit, bool fill = false) {
   h->last_ms = ms;
 }
 
-std::string readback(hvm_gpu *h, u64 root_cell) {
+// Host::readback (readback.rs:23-33): the root's tree, then each pending
+// redex as "\n  & a ~ b" (the Display of ast::Net, ast.rs:234-244), with one
+// variable naming across all of them.  with_redexes: the net's pending
+// redexes are in the global ring (a single net after reduce(limit)).
+std::string readback(hvm_gpu *h, u64 root_cell, bool with_redexes = false) {
+  u64 q_head = 0, q_tail = 0;
+  if (with_redexes) {
+    Globals g;
+    hip_check(hipMemcpy(&g, h->g, sizeof(Globals), hipMemcpyDeviceToHost), "read globals");
+    q_head = g.q_head;
+    q_tail = std::max(g.q_head, g.q_tail);
+    if (q_tail - q_head > h->qslots) throw Error(HVM_E_READBACK, "readback: redex ring inconsistent");
+  }
   for (;;) {
     if (!h->d_rb_out) {
       h->rb_cap = std::max<u64>(h->rb_cap, 1 << 20);
@@ -288,7 +300,8 @@ std::string readback(hvm_gpu *h, u64 root_cell) {
       hip_check(hipMalloc(&h->d_rb_stack, h->rb_cap * sizeof(u64)), "hipMalloc readback");
     }
     Params P = make_params(h, ~0ull);
-    hip_check(launch_readback(P, root_cell, h->d_rb_stack, h->rb_cap, h->d_rb_out, h->rb_cap, h->stream),
+    hip_check(launch_readback(P, root_cell, q_head, q_tail, h->d_rb_stack, h->rb_cap, h->d_rb_out, h->rb_cap,
+                              h->stream),
               "launch readback");
     hip_check(hipStreamSynchronize(h->stream), "readback");
     Globals g;
@@ -311,6 +324,11 @@ std::string readback(hvm_gpu *h, u64 root_cell) {
       std::map<u64, u64> vars;
       u64 next = 0;
       text = hvm64::readback_text(h->host, stream, pos, vars, next);
+      for (u64 r = q_head; r < q_tail; r++) {
+        const std::string a = hvm64::readback_text(h->host, stream, pos, vars, next);
+        const std::string b = hvm64::readback_text(h->host, stream, pos, vars, next);
+        text += "\n  & " + a + " ~ " + b;
+      }
     });
     return text;
   }
@@ -1083,7 +1101,7 @@ int hvm_gpu_snapshot(hvm_gpu *h, hvm_net_image *img, uint64_t first, uint64_t n,
 int hvm_gpu_readback(hvm_gpu *h, char *buf, uint64_t cap, uint64_t *len) {
   int rc = 0;
   int g = guarded(h, [&] {
-    std::string text = readback(h, h->root_cell);
+    std::string text = readback(h, h->root_cell, true);
     if (len) *len = text.size();
     if (!buf || cap < text.size() + 1) {
       rc = HVM_E_TRUNC;

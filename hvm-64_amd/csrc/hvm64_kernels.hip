@@ -696,48 +696,61 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
 }
 
 // Serialise the tree hanging from `root_cell` in preorder (readback.rs:23-87
-// walk order: aux port 1 before aux port 2).  Principal ports are emitted as
-// they are; a wire between two aux ports is emitted as Var(cell) of the var
-// cell that joins them.  g->result = number of entries, or ERR_READBACK.
-__global__ void k_readback(Params P, u64 root_cell, u64 *stack, u64 stack_cap, u64 *out, u64 out_cap) {
+// walk order: aux port 1 before aux port 2), then the two sides of each
+// pending redex in global-ring slots [q_head, q_tail) (after a reduce(limit),
+// every pending redex is there).  Principal ports are emitted as they are; a
+// wire between two aux ports is emitted as Var(cell) of the var cell that
+// joins them.  g->result = number of entries, or ERR_READBACK.
+__global__ void k_readback(Params P, u64 root_cell, u64 q_head, u64 q_tail, u64 *stack, u64 stack_cap, u64 *out,
+                           u64 out_cap) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  u64 sp = 0, n = 0;
-  stack[sp++] = var_port(root_cell);
-  while (sp > 0) {
-    u64 p = stack[--sp];
-    u64 hops = 0;
-    while (is_endpoint(p)) {
-      u64 v = ld(P, paddr(p));
-      if (v == P_NONE) break;
-      if (v == P_FREE || ++hops > (1ull << 24)) {
-        P.g->err_info = 2;  // dangling or cyclic wire
+  u64 n = 0;
+  // the root's tree, then both sides of every pending redex in ring order
+  // (Host::readback reads rt_net.redexes after the root, readback.rs:28-30)
+  for (u64 k = 0; k < 1 + 2 * (q_tail - q_head); k++) {
+    u64 sp = 0;
+    if (k == 0) {
+      stack[sp++] = var_port(root_cell);
+    } else {
+      const u64 *slot = P.queue + ((q_head + (k - 1) / 2) & P.qmask) * 2;
+      stack[sp++] = gld((u64 *)slot + ((k - 1) & 1));
+    }
+    while (sp > 0) {
+      u64 p = stack[--sp];
+      u64 hops = 0;
+      while (is_endpoint(p)) {
+        u64 v = ld(P, paddr(p));
+        if (v == P_NONE) break;
+        if (v == P_FREE || ++hops > (1ull << 24)) {
+          P.g->err_info = 2;  // dangling or cyclic wire
+          P.g->err = ERR_READBACK;
+          P.g->result = n;
+          return;
+        }
+        p = v;
+      }
+      if (n >= out_cap) {
+        P.g->err_info = 1;  // buffer too small
         P.g->err = ERR_READBACK;
         P.g->result = n;
         return;
       }
-      p = v;
-    }
-    if (n >= out_cap) {
-      P.g->err_info = 1;  // buffer too small
-      P.g->err = ERR_READBACK;
-      P.g->result = n;
-      return;
-    }
-    if (is_endpoint(p)) {
-      out[n++] = var_port(paddr(p));
-      continue;
-    }
-    out[n++] = p;
-    if (is_binary(p)) {
-      if (sp + 2 > stack_cap) {
-        P.g->err_info = 1;
-        P.g->err = ERR_READBACK;
-        P.g->result = n;
-        return;
+      if (is_endpoint(p)) {
+        out[n++] = var_port(paddr(p));
+        continue;
       }
-      u64 a = paddr(p);
-      stack[sp++] = ld(P, a + 8);
-      stack[sp++] = ld(P, a);
+      out[n++] = p;
+      if (is_binary(p)) {
+        if (sp + 2 > stack_cap) {
+          P.g->err_info = 1;
+          P.g->err = ERR_READBACK;
+          P.g->result = n;
+          return;
+        }
+        u64 a = paddr(p);
+        stack[sp++] = ld(P, a + 8);
+        stack[sp++] = ld(P, a);
+      }
     }
   }
   P.g->result = n;
@@ -824,9 +837,10 @@ hipError_t launch_reduce(const Params &P, u32 blocks, hipStream_t st) {
   else hipLaunchKernelGGL(k_reduce<false>, dim3(blocks), dim3(WAVES_PER_BLOCK * 64), 0, st, P);
   return hipGetLastError();
 }
-hipError_t launch_readback(const Params &P, u64 root_cell, u64 *stack, u64 stack_cap, u64 *out, u64 out_cap,
-                           hipStream_t st) {
-  hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, st, P, root_cell, stack, stack_cap, out, out_cap);
+hipError_t launch_readback(const Params &P, u64 root_cell, u64 q_head, u64 q_tail, u64 *stack, u64 stack_cap,
+                           u64 *out, u64 out_cap, hipStream_t st) {
+  hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, st, P, root_cell, q_head, q_tail, stack, stack_cap, out,
+                     out_cap);
   return hipGetLastError();
 }
 u32 reduce_waves_per_block() { return WAVES_PER_BLOCK; }

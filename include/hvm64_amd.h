@@ -213,7 +213,10 @@ int hvm_gpu_rewrites(hvm_gpu *h, uint64_t out[5]);
 int hvm_gpu_shortcuts(hvm_gpu *h, uint64_t out[3]);
 
 /* ~ Host::readback(&net) + Display (host/src/readback.rs:23-33,
- * ast/src/ast.rs:234-244): normal form as text.  If `cap` is too small,
+ * ast/src/ast.rs:234-244): the net as text, the root's tree followed by
+ * "\n  & a ~ b" for each pending redex (none once the net is normal; after
+ * reduce(limit), the redexes still to reduce, in the device's ring order,
+ * which differs from the reference's queue order).  If `cap` is too small,
  * returns HVM_E_TRUNC with the needed length (without NUL) in *len. */
 int hvm_gpu_readback(hvm_gpu *h, char *buf, uint64_t cap, uint64_t *len);
 

@@ -493,6 +493,53 @@ def test_reduce_limit_bounds_pops(gpu_heap, oracle, limit):
     assert net.rwts.as_dict() == ref["rwts"]
 
 
+def _with_main(book, body):
+    """The book with @main's definition (its tree and its redex lines)
+    replaced by `body`."""
+    out, skip = [], False
+    for line in book.splitlines():
+        if line.startswith("@"):
+            skip = re.match(r"@main\s*=", line) is not None
+            if skip:
+                out.append("@main = " + body)
+                continue
+        if not skip:
+            out.append(line)
+    return "\n".join(out) + "\n"
+
+
+@pytest.mark.parametrize("limit", [1, 1000, 20_000])  # (47,181 pops in all)
+def test_readback_after_reduce_limit(gpu_heap, oracle, limit):
+    """Host::readback after Net::reduce(limit) (readback.rs:28-30): the root's
+    tree and every pending redex as "& a ~ b".  Which redexes are pending
+    depends on the order of reduction, so the text is checked by what it
+    must mean: booted in place of @main, the partial net reduces on the
+    oracle to the full program's normal form, and the counts of the GPU's
+    first part plus the oracle's rest equal the oracle's counts for the
+    whole (less the one DREF of booting the partial net)."""
+    import hvm64
+    from hvm64 import configs
+    text = configs.bitonic(7, 3)
+    print("case readback after reduce", limit, flush=True)
+    host = hvm64.Host(text, gpu_heap)
+    net = hvm64.Net(gpu_heap)
+    net.boot(host.defs["main"])
+    assert net.reduce(limit) is None
+    part = host.readback(net)
+    assert "\n  & " in part
+    done = net.rwts.as_dict()
+    full = oracle.run(text, heap_words=1 << 24)
+    rest = oracle.run(_with_main(text, part), heap_words=1 << 24)
+    assert rest["nf"] == full["nf"]
+    for k in ["anni", "comm", "eras", "oper"]:
+        assert done[k] + rest["rwts"][k] == full["rwts"][k], k
+    assert done["dref"] + rest["rwts"]["dref"] - 1 == full["rwts"]["dref"]
+    # and the net itself still reaches the normal form with the full counts
+    net.normal()
+    assert host.readback(net) == full["nf"]
+    assert net.rwts.as_dict() == full["rwts"]
+
+
 # ---------------------------------------------------------------------------
 # interpreted and compiled definitions give the same results
 # ---------------------------------------------------------------------------
