@@ -356,11 +356,12 @@ def batch_cpu_baseline(book, table, nets, seed, sample=256):
 
 def bench_serve(args):
     """Config 5: seeded Poisson arrivals of the tree-sum / bitonic mix (sizes
-    log-uniform), continuous batching over `slots` concurrent heaps; reports
-    p50/p99 latency and aggregate MIPS."""
+    log-uniform), served by one persistent launch that admits requests into
+    free slot arenas itself (hvm_gpu_serve_*); reports p50/p99 latency
+    (arrival to the host poll that saw the result) and aggregate MIPS."""
     from hvm64 import serve
     heap_bytes = int(args.heap_gb * (1 << 30))
-    srv = serve.Server(slots=args.slots, heap_bytes=heap_bytes, max_batch=args.max_batch, book_seed=args.seed)
+    srv = serve.Server(slots=args.slots, heap_bytes=heap_bytes, book_seed=args.seed)
     try:
         if args.warmup:
             srv.run(serve.poisson_requests(args.rate, 0.2, seed=args.seed + 1))
@@ -369,7 +370,7 @@ def bench_serve(args):
     finally:
         srv.close()
     out = {
-        "metric": "p99 latency on 1 MI355X: serving (Poisson arrivals, continuous batching)",
+        "metric": "p99 latency on 1 MI355X: serving (Poisson arrivals, admission inside the running kernel)",
         "value": round(st.p99_ms, 2),
         "unit": "ms",
         "n_gpus": 1,
@@ -384,8 +385,7 @@ def bench_serve(args):
         "config": {"workload": f"Poisson arrivals at {args.rate:g}/s for {args.duration:g} s of tree-sum depth / "
                                f"bitonic k log-uniform over [8,16] / [6,12] (seed {args.seed})",
                    "rate": args.rate, "duration_s": args.duration, "slots": st.slots,
-                   "blocks_per_slot": st.per_slot_blocks, "heap_gb_per_slot": args.heap_gb,
-                   "max_batch": args.max_batch},
+                   "slot_nodes": srv.slot_nodes, "heap_gb": args.heap_gb, "grid_blocks": st.per_slot_blocks},
         "p50_ms": round(st.p50_ms, 2),
         "p99_ms": round(st.p99_ms, 2),
         "max_ms": round(st.max_ms, 2),
@@ -393,10 +393,11 @@ def bench_serve(args):
         "requests": st.requests,
         "completed": st.completed,
         "result_ok": st.all_ok,
-        "batches": st.batches,
-        "mean_batch": round(st.mean_batch, 1),
+        "submits": st.batches,
+        "mean_submit": round(st.mean_batch, 1),
         "rewrites": st.rewrites,
         "wall_s": round(st.wall_s, 3),
+        "kernel_ms": round(st.kernel_ms, 3),
     }
     print(json.dumps(out), flush=True)
     return 0 if st.all_ok else 1
@@ -411,16 +412,14 @@ def main():
     ap.add_argument("--nets", type=int, default=4096, help="batch: number of nets")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve: Poisson arrivals per second")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
-    ap.add_argument("--slots", type=int, default=4, help="serve: concurrent heaps (each with 1/slots of the grid)")
-    ap.add_argument("--max-batch", type=int, default=1024, help="serve: requests admitted per batch at most")
+    ap.add_argument("--slots", type=int, default=1024, help="serve: request arenas (requests in flight at most)")
     ap.add_argument("--per-net-sample", type=int, default=256,
                     help="batch: nets timed with one launch sequence each (the per-net baseline)")
     ap.add_argument("--no-tune", action="store_true", help="batch: keep the default launch tuning")
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--heap-gb", type=float, default=None,
-                    help="heap size (default 192 GiB of the MI355X's 288 GB, batch 224 GiB; serve: split over the "
-                         "slots)")
+                    help="heap size (default 192 GiB of the MI355X's 288 GB, batch 224 GiB)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--interpreted", action="store_true",
                     help="run every definition interpreted (compiled definitions off)")
@@ -434,7 +433,7 @@ def main():
         # 32 GiB and ~1890 with 192 GiB
         # batch: 4,096 arenas hold ~181 GiB (every node a net allocates,
         # plus the ends of lanes' chunks); the pool grows them
-        args.heap_gb = 192.0 / args.slots if args.workload == "serve" else 224.0 if args.workload == "batch" else 192.0
+        args.heap_gb = 224.0 if args.workload == "batch" else 192.0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs one net per "
               f"device and this build has no multi-GPU path).", file=sys.stderr)

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -88,6 +89,17 @@ struct hvm_gpu {
   u32 idle_words = 0;
   Globals last{};
   double last_ms = 0;
+  // the serving launch (hvm_gpu_serve_start .. _stop)
+  struct Serve {
+    bool running = false;
+    u32 slots = 0;
+    u64 *dev = nullptr;        // pending | slot_root | slot_tag | free_ring | ctl | words | base0 | len | gmap | inl
+    u64 *host = nullptr;       // pinned: host_ctl | requests | completions
+    u64 *host_dev = nullptr;   // its device address
+    u64 req_cap = 0, done_cap = 0;
+    u64 req_tail = 0, done_read = 0;
+    u64 *req = nullptr, *done = nullptr, *ctl = nullptr;  // host views
+  } serve;
   bool heap_dirty = false;  // arenas left words the allocation rings would read as live
   u64 timeout_ticks = 0;  // wall-clock ticks a reduction may run (0 = unbounded)
   u64 wall_khz = 100000;
@@ -174,7 +186,11 @@ void sync_globals(hvm_gpu *h) {
                       : e == ERR_TIMEOUT ? "reduction exceeded its time limit"
                       : e == ERR_BADPTR ? "invalid heap or book address (engine bug)"
                                           : "device error";
-    if (e == ERR_BADPTR) throw Error(code, std::string(msg) + ", check " + std::to_string(h->last.err_info));
+    if (e == ERR_BADPTR) {
+      char what[32];
+      snprintf(what, sizeof what, "0x%llx", (unsigned long long)h->last.result);
+      throw Error(code, std::string(msg) + ", check " + std::to_string(h->last.err_info) + " (" + what + ")");
+    }
     if (e == ERR_OOM && h->last.err_info == OOM_POOL) throw Error(code, "OOM: the arena pool ran dry");
     if (e == ERR_OOM && h->last.err_info == OOM_OVERFLOW) throw Error(code, "OOM: the arena overflow region ran dry");
     throw Error(code, msg);
@@ -348,8 +364,12 @@ u64 *batch_buffer(hvm_gpu *h, u64 words) {
 // copies and launches made from a thread other than the one that opened the
 // handle must land on the handle's GPU.
 template <class F>
-int guarded(hvm_gpu *h, F fn) {
+int guarded(hvm_gpu *h, F fn, bool while_serving = false) {
   if (!h) return HVM_E_ARG;
+  if (h->serve.running && !while_serving) {
+    h->err = "the handle is serving (hvm_gpu_serve_stop first)";
+    return HVM_E_ARG;
+  }
   try {
     hip_check(hipSetDevice(h->device), "hipSetDevice");
     fn();
@@ -676,6 +696,188 @@ int hvm_gpu_batch_tune(hvm_gpu *h, hvm_batch_tuning *best) {
   });
 }
 
+int hvm_gpu_serve_start(hvm_gpu *h, uint32_t slots, uint64_t slot_nodes, uint64_t queue) {
+  return guarded(h, [&] {
+    if (slots == 0 || slot_nodes == 0 || queue == 0) throw Error(HVM_E_ARG, "empty serving configuration");
+    if (slots > (1u << 20)) throw Error(HVM_E_ARG, "too many slots");
+    if (!h->ndefs) throw Error(HVM_E_ARG, "no book loaded");
+    drop_plan(h);
+    hvm_gpu::Serve &sv = h->serve;
+    sv = hvm_gpu::Serve{};
+    const u64 heap_nodes = h->heap_words / 2;
+    u32 gs = 10;
+    while ((heap_nodes >> gs) > (4u << 20)) gs++;
+    const u64 gmap_len = heap_nodes >> gs;
+    const u64 slot_g = (slot_nodes + (1ull << gs) - 1) >> gs;
+    const u64 first = (RESERVED_WORDS / 2 + (1ull << gs) - 1) >> gs;  // granule 0: the fixed cells
+    if (slot_g > 0xFFFFFFFFull || first + (u64)slots * slot_g > gmap_len)
+      throw Error(HVM_E_OOM, "the slots' arenas exceed the heap");
+    u64 rq = 1, dq = 1;
+    while (rq < queue) rq <<= 1;
+    while (dq < 4 * rq) dq <<= 1;
+    sv.slots = slots;
+    sv.req_cap = rq;
+    sv.done_cap = dq;
+    // device: pending | root | tag | free ring | counters | words | base0 | len | gmap | inl
+    const u64 inl_words = (u64)h->waves * 64 * SERVE_INL * 2;
+    const u64 words = (slots + 1) / 2 + 4 * (u64)slots + SV_WORDS + (u64)slots + (slots + 1) / 2 + (gmap_len + 1) / 2 +
+                      inl_words + 64;
+    hip_check(hipMalloc(&sv.dev, words * 8), "hipMalloc serving");
+    u64 *w = sv.dev;
+    u32 *pending = (u32 *)w;
+    w += (slots + 1) / 2;
+    u64 *slot_root = w;
+    w += slots;
+    u64 *slot_tag = w;
+    w += slots;
+    u64 *free_ring = w;
+    w += slots;
+    u64 *ctl = w;
+    w += SV_WORDS;
+    u64 *word = w;
+    w += slots;
+    u64 *base0 = w;
+    w += slots;
+    u32 *len = (u32 *)w;
+    w += (slots + 1) / 2;
+    u32 *gmap = (u32 *)w;
+    w += (gmap_len + 1) / 2;
+    u64 *inl = w;
+    w += inl_words;
+    u64 *nopool = w;  // the serving arenas do not grow: a pool and an overflow region of 0
+    std::vector<u64> h_word(slots), h_base0(slots), h_free(slots), h_ctl(SV_WORDS, 0);
+    std::vector<u32> h_len(slots, (u32)slot_g), h_gmap(gmap_len, NO_NET_HOST);
+    for (u32 i = 0; i < slots; i++) {
+      h_base0[i] = first + (u64)i * slot_g;
+      h_word[i] = h_base0[i] << 32;
+      h_free[i] = (u64)i + 1;
+      for (u64 g = h_base0[i]; g < h_base0[i] + slot_g; g++) h_gmap[g] = i;
+    }
+    h_ctl[SV_FREE_TAIL] = slots;
+    h_ctl[SV_FREE_COUNT] = slots;
+    hip_check(hipMemset(pending, 0, (size_t)slots * 4), "init serving");
+    hip_check(hipMemcpy(free_ring, h_free.data(), (size_t)slots * 8, hipMemcpyHostToDevice), "init serving");
+    hip_check(hipMemcpy(ctl, h_ctl.data(), SV_WORDS * 8, hipMemcpyHostToDevice), "init serving");
+    hip_check(hipMemcpy(word, h_word.data(), (size_t)slots * 8, hipMemcpyHostToDevice), "init serving");
+    hip_check(hipMemcpy(base0, h_base0.data(), (size_t)slots * 8, hipMemcpyHostToDevice), "init serving");
+    hip_check(hipMemcpy(len, h_len.data(), (size_t)slots * 4, hipMemcpyHostToDevice), "init serving");
+    hip_check(hipMemcpy(gmap, h_gmap.data(), gmap_len * 4, hipMemcpyHostToDevice), "init serving");
+    hip_check(hipMemset(nopool, 0, 64 * 8), "init serving");
+    // pinned host memory: control words | requests | completions
+    const u64 hwords = SH_WORDS + 2 * rq + 4 * dq;
+    hip_check(hipHostMalloc((void **)&sv.host, hwords * 8, hipHostMallocMapped | hipHostMallocCoherent),
+              "hipHostMalloc serving rings");
+    std::memset(sv.host, 0, hwords * 8);
+    hip_check(hipHostGetDevicePointer((void **)&sv.host_dev, sv.host, 0), "serving rings' device address");
+    sv.ctl = sv.host;
+    sv.req = sv.host + SH_WORDS;
+    sv.done = sv.req + 2 * rq;
+    // a clean control block, idle set, mailboxes and global ring
+    hip_check(launch_fill(h->queue, h->qslots * 2, 0, h->stream), "clear ring");
+    hip_check(hipMemsetAsync(h->idle_bits, 0, (size_t)h->idle_words * 8, h->stream), "clear idle set");
+    hip_check(hipMemsetAsync(h->mbox, 0, (size_t)h->waves * MBOX_WORDS * 8, h->stream), "clear mailboxes");
+    Globals g0{};
+    hip_check(hipMemcpyAsync(h->g, &g0, sizeof(Globals), hipMemcpyHostToDevice, h->stream), "init globals");
+    hip_check(hipStreamSynchronize(h->stream), "init serving");
+    h->last = g0;
+    h->heap_dirty = true;
+    Params P = make_params(h, ~0ull);
+    P.fill = 1;
+    P.gmap = gmap;
+    P.gmap_len = gmap_len;
+    P.gran_shift = gs;
+    P.arena_word = word;
+    P.arena_base0 = base0;
+    P.arena_len = len;
+    P.pool_next = nopool;
+    P.pool_end = 0;
+    P.ov_next = nopool + 16;
+    P.ov_end = 0;
+    P.n_nets = slots;
+    P.refill_granules = 1;
+    P.arena_chunk = 32;
+    P.serve = 1;
+    P.serve_slots = slots;
+    P.pending = pending;
+    P.slot_root = slot_root;
+    P.slot_tag = slot_tag;
+    P.free_ring = free_ring;
+    P.serve_ctl = ctl;
+    P.host_ctl = sv.host_dev;
+    P.req = sv.host_dev + SH_WORDS;
+    P.req_mask = rq - 1;
+    P.done = sv.host_dev + SH_WORDS + 2 * rq;
+    P.done_mask = dq - 1;
+    P.inl = inl;
+    hip_check(hipEventRecord(h->ev0, h->stream), "event");
+    hip_check(launch_reduce(P, h->blocks, h->stream), "launch serving kernel");
+    hip_check(hipEventRecord(h->ev1, h->stream), "event");
+    sv.running = true;
+  });
+}
+
+int hvm_gpu_serve_submit(hvm_gpu *h, uint32_t n, const uint32_t *def_ids, const uint64_t *tags) {
+  return guarded(h, [&] {
+    hvm_gpu::Serve &sv = h->serve;
+    if (!sv.running) throw Error(HVM_E_ARG, "not serving");
+    if (n && (!def_ids || !tags)) throw Error(HVM_E_ARG, "null argument");
+    const u64 dev_err = __atomic_load_n(&sv.ctl[SH_ERR], __ATOMIC_ACQUIRE);
+    if (dev_err) throw Error(HVM_E_DEVICE, "the serving kernel stopped with device error " + std::to_string(dev_err));
+    for (u32 i = 0; i < n; i++)
+      if (def_ids[i] >= h->ndefs) throw Error(HVM_E_ARG, "no such definition");
+    const u64 head = __atomic_load_n(&sv.ctl[SH_REQ_HEAD], __ATOMIC_ACQUIRE);
+    if (sv.req_tail - head + n > sv.req_cap) throw Error(HVM_E_QUEUE, "request ring full");
+    for (u32 i = 0; i < n; i++) {
+      u64 *e = sv.req + ((sv.req_tail + i) & (sv.req_cap - 1)) * 2;
+      e[0] = (u64)def_ids[i] + 1;
+      e[1] = tags[i];
+    }
+    sv.req_tail += n;
+    __atomic_store_n(&sv.ctl[SH_REQ_TAIL], sv.req_tail, __ATOMIC_RELEASE);
+  }, true);
+}
+
+int hvm_gpu_serve_poll(hvm_gpu *h, uint64_t *tags, uint64_t *ports, uint32_t cap, uint32_t *n) {
+  return guarded(h, [&] {
+    hvm_gpu::Serve &sv = h->serve;
+    if (!sv.running) throw Error(HVM_E_ARG, "not serving");
+    if (!n || (cap && (!tags || !ports))) throw Error(HVM_E_ARG, "null argument");
+    u32 k = 0;
+    while (k < cap) {
+      const u64 *e = sv.done + (sv.done_read & (sv.done_cap - 1)) * 4;
+      if (__atomic_load_n(&e[0], __ATOMIC_ACQUIRE) != sv.done_read + 1) break;
+      tags[k] = e[1];
+      ports[k] = e[2];
+      k++;
+      sv.done_read++;
+    }
+    __atomic_store_n(&sv.ctl[SH_DONE_READ], sv.done_read, __ATOMIC_RELEASE);
+    *n = k;
+    const u64 dev_err = __atomic_load_n(&sv.ctl[SH_ERR], __ATOMIC_ACQUIRE);
+    if (!k && dev_err)
+      throw Error(HVM_E_DEVICE, "the serving kernel stopped with device error " + std::to_string(dev_err));
+  }, true);
+}
+
+int hvm_gpu_serve_stop(hvm_gpu *h, double *ms) {
+  return guarded(h, [&] {
+    hvm_gpu::Serve &sv = h->serve;
+    if (!sv.running) throw Error(HVM_E_ARG, "not serving");
+    __atomic_store_n(&sv.ctl[SH_STOP], 1ull, __ATOMIC_RELEASE);
+    const hipError_t e = hipStreamSynchronize(h->stream);
+    float k_ms = 0;
+    (void)hipEventElapsedTime(&k_ms, h->ev0, h->ev1);
+    h->last_ms = k_ms;
+    if (ms) *ms = k_ms;
+    sv.running = false;
+    if (sv.host) (void)hipHostFree(sv.host);
+    dfree(sv.dev);
+    sv = hvm_gpu::Serve{};
+    hip_check(e, "serving kernel");
+    sync_globals(h);
+  }, true);
+}
+
 int hvm_gpu_alloc_count(hvm_gpu *h, uint64_t *nodes) {
   return guarded(h, [&] {
     if (!nodes) throw Error(HVM_E_ARG, "null argument");
@@ -767,6 +969,7 @@ int hvm_gpu_open_grid(int device, uint64_t heap_bytes, uint32_t max_blocks, hvm_
 void hvm_gpu_close(hvm_gpu *h) {
   if (!h) return;
   (void)hipSetDevice(h->device);
+  if (h->serve.running) (void)hvm_gpu_serve_stop(h, nullptr);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   drop_plan(h);
   dfree(h->heap);
@@ -908,7 +1111,7 @@ int hvm_gpu_def_id(hvm_gpu *h, const char *name, uint32_t *id) {
   return guarded(h, [&] {
     if (!name || !id) throw Error(HVM_E_ARG, "null argument");
     *id = h->host.def_id(name);
-  });
+  }, true);  // host-only: allowed while serving
 }
 
 int hvm_gpu_reset(hvm_gpu *h) {
@@ -1250,7 +1453,7 @@ int hvm_gpu_grid(hvm_gpu *h, uint32_t *blocks, uint32_t *waves) {
   return guarded(h, [&] {
     if (blocks) *blocks = h->blocks;
     if (waves) *waves = h->waves;
-  });
+  }, true);
 }
 
 }  // extern "C"

@@ -78,7 +78,8 @@ enum : u32 {
 enum : u32 { OOM_POOL = 1, OOM_OVERFLOW = 2 };
 
 // Which consistency check reported ERR_BADPTR (Globals::err_info).
-enum : u32 { BAD_HEAP_ADDR = 1, BAD_LINK_LIST, BAD_CHAIN, BAD_RESOLVE, BAD_SLOTS, BAD_DEF_CALL, BAD_DEF_EXPAND };
+enum : u32 { BAD_HEAP_ADDR = 1, BAD_LINK_LIST, BAD_CHAIN, BAD_RESOLVE, BAD_SLOTS, BAD_DEF_CALL, BAD_DEF_EXPAND,
+             BAD_NET };  // BAD_NET: a port beyond the granule map
 
 // Scheduler statistics (summed over waves): iterations with work, idle
 // iterations, wall-clock ticks (100 MHz) spent in iterations with work,
@@ -177,7 +178,36 @@ struct Params {
   u32 refill_granules; // largest refill extent (granules)
   u32 arena_chunk;     // nodes a lane takes from its net's arena while it stays on the net
   u32 gran_shift;
+  // Serving (serve != 0): the arenas are request slots (net = slot), reused
+  // once a slot's net is normal.  pending[s] counts the slot's redexes not
+  // yet consumed; the task that brings it to 0 reports the net's root port.
+  // Request and completion rings live in pinned host memory.
+  u32 serve;
+  u32 serve_slots;
+  u32 *pending;
+  u64 *slot_root;      // root cell of each slot's net
+  u64 *slot_tag;       // the request's tag
+  u64 *free_ring;      // slot + 1 entries (0: not yet written), serve_slots entries
+  u64 *serve_ctl;      // device counters, each on its own 128-byte line (SV_*)
+  const u64 *req;      // host: requests, (definition + 1, tag) pairs
+  u64 req_mask;        // request ring entries - 1
+  u64 *host_ctl;       // host: control words, each on its own 128-byte line (SH_*)
+  u64 *done;           // host: completions, (seq, tag, port, 0) per entry
+  u64 done_mask;       // completion ring entries - 1
+  u64 *inl;            // per lane: SERVE_INL pairs of redexes run inline
 };
+
+// Device counters of a serving launch (Params::serve_ctl, in words).
+enum : u32 { SV_FREE_HEAD = 0, SV_FREE_TAIL = 16, SV_FREE_COUNT = 32, SV_REQ_HEAD = 48, SV_DONE_TAIL = 64,
+             SV_WORDS = 80 };
+// Host-side control words of a serving launch (Params::host_ctl, in words):
+// requests written (host), requests admitted (device), completions read
+// (host), stop flag (host), device error code (device).
+enum : u32 { SH_REQ_TAIL = 0, SH_REQ_HEAD = 16, SH_DONE_READ = 32, SH_STOP = 48, SH_ERR = 64, SH_WORDS = 80 };
+// Redexes between two ports that reach no heap node carry no net; a serving
+// lane runs them at once, inside the task that made them (at most this many
+// pending per lane).
+static const u32 SERVE_INL = 32;
 
 // Mailbox of a wave: a count word on its own 128-byte line, then up to
 // MBOX_PAIRS redex pairs handed to the wave by another wave.

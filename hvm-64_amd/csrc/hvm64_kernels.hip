@@ -264,9 +264,78 @@ DEV u64 spill_to_global(const Params &P, u64 *bag, u32 *bot_c, u32 c, u32 cnt, u
   return __shfl(old, 0, 64) + ((u64)cnt << 32);
 }
 
+// Serving: wave 0 admits waiting requests, one per lane (at most 64, and at
+// most as many as there are free slots), booting each into a free slot.
+// Returns how many it booted.
+DEV u32 serve_admit(const Params &P, Lane &L, u32 lane, u64 t_end) {
+  u64 *const sv = P.serve_ctl;
+  u64 *const hc = P.host_ctl;
+  u64 k = 0, head = 0, fhead = 0;
+  if (lane == 0) {
+    const u64 tail = HVM_LD_SYS(&hc[SH_REQ_TAIL]);
+    head = HVM_LD(&sv[SV_REQ_HEAD]);  // only wave 0 moves the admission counters
+    const u64 avail = tail > head ? tail - head : 0;
+    const u64 free = HVM_LD(&sv[SV_FREE_COUNT]);
+    k = avail < free ? avail : free;
+    k = k < 64 ? k : 64;
+    if (k) {
+      HVM_FAA(&sv[SV_FREE_COUNT], (u64)0 - k);
+      fhead = HVM_LD(&sv[SV_FREE_HEAD]);
+      HVM_ST(&sv[SV_FREE_HEAD], fhead + k);
+      HVM_ST(&sv[SV_REQ_HEAD], head + k);
+    }
+  }
+  k = __shfl(k, 0, 64);
+  if (k == 0) return 0;
+  head = __shfl(head, 0, 64);
+  fhead = __shfl(fhead, 0, 64);
+  if (lane < k) {
+    // the entry a completing lane wrote before it counted the slot free
+    u64 *slot = &P.free_ring[(fhead + lane) % P.serve_slots];
+    u64 v = 0;
+    for (;;) {
+      v = HVM_LD(slot);
+      if (v != 0) break;
+      if (wall_clock64() > t_end) {
+        fail(L, ERR_TIMEOUT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (v != 0) {
+      HVM_ST(slot, 0);
+      const u64 *rq = P.req + ((head + lane) & P.req_mask) * 2;
+      const u64 d = HVM_LD_SYS(rq), tag = HVM_LD_SYS(rq + 1);
+      if (d == 0 || d - 1 >= P.ndefs) fail(L, ERR_BADPTR, BAD_DEF_CALL);
+      else serve_boot(L, (u32)(v - 1), (u32)(d - 1), tag);
+    }
+  }
+  if (lane == 0) HVM_ST_SYS(&hc[SH_REQ_HEAD], head + k);  // the host sees the ring's room
+  return (u32)k;
+}
+
+// Serving: requests waiting and slots free (wave 0, lane 0's view).
+DEV bool serve_waiting(const Params &P, u32 lane) {
+  u64 w = 0;
+  if (lane == 0) {
+    const u64 tail = HVM_LD_SYS(&P.host_ctl[SH_REQ_TAIL]);
+    w = tail > HVM_LD(&P.serve_ctl[SV_REQ_HEAD]) && HVM_LD(&P.serve_ctl[SV_FREE_COUNT]) > 0;
+  }
+  return __shfl(w, 0, 64) != 0;
+}
+
+// After pushes outside an iteration (admission), trim each ring whose push
+// counter ran past its capacity: those pushes went to the global ring.
+DEV void clamp_rings(u32 *top_c, const u32 *bot_c, u32 lane) {
+  if (lane < NCLS && top_c[lane] - bot_c[lane] > ring_cap(lane)) top_c[lane] = bot_c[lane] + ring_cap(lane);
+  __builtin_amdgcn_wave_barrier();
+}
+
 // ARENAS: compiled for per-net arenas (P.gmap set) or for the per-lane
 // allocation rings; each instantiation carries only its mode's allocator.
-template <bool ARENAS>
+// SERVE: the serving launch (arenas are request slots; the kernel runs until
+// the host sets the stop flag and no work is left).
+template <bool ARENAS, bool SERVE>
 __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   __shared__ u64 s_bag[WAVES_PER_BLOCK][RING_PAIRS * 2];
   __shared__ u64 s_T[WAVES_PER_BLOCK][(LDS_SLOTS + 2 * LINK_CAP) * 64];
@@ -299,6 +368,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   Lane L;
   lane_init(L, P, gwave, lane);
   L.arenas = ARENAS;
+  L.serve = SERVE;
   L.T_lds = &s_T[wib][lane];
   L.PL = &s_T[wib][LDS_SLOTS * 64 + lane];
   if (P.book_lds) {
@@ -330,6 +400,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   u64 *my_box = use_mbox ? P.mbox + (u64)gwave * MBOX_WORDS : nullptr;
   bool in_set = false;         // this wave's bit is set in P.idle_bits
   u32 idle_polls = 0;
+  u32 admit_tick = 0;          // serving, wave 0: busy iterations since the last admission
   if (use_mbox) {  // every wave starts idle: in the idle set, not counted in S
     if (lane == 0)
       __hip_atomic_fetch_or(&P.idle_bits[gwave >> 6], 1ull << (gwave & 63), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -341,11 +412,21 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
       S_cache = __shfl(S_next, 0, 64);
       u32 err = __shfl(err_next, 0, 64);
       fresh = false;
-      if (err) break;
+      if (err) {
+        if (SERVE && lane == 0) HVM_ST_SYS(&P.host_ctl[SH_ERR], (u64)err);
+        break;
+      }
       if (wall_clock64() > t_end) {
         fail(L, ERR_TIMEOUT);
         break;
       }
+    }
+    // ---- serving: wave 0 admits requests every fourth busy iteration
+    if (SERVE && gwave == 0 && active && (++admit_tick & 3) == 0) {
+      __builtin_amdgcn_wave_barrier();
+      serve_admit(P, L, lane, t_end);
+      __builtin_amdgcn_wave_barrier();
+      clamp_rings(top_c, bot_c, lane);
     }
     // ring occupancy: lane c < NCLS reads class c
     u32 my_top = lane < NCLS ? top_c[lane] : 0, my_bot = lane < NCLS ? bot_c[lane] : 0;
@@ -371,7 +452,15 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
         S_cache = __shfl(old, 0, 64) - 1;
         active = false;
         idle_polls = 0;
-        if (S_cache == 0) {  // nothing active, nothing queued: done
+        // nothing active, nothing queued: done (serving: once the host has
+        // asked to stop)
+        bool end = S_cache == 0;
+        if (SERVE && end) {
+          u64 stop = 0;
+          if (lane == 0) stop = HVM_LD_SYS(&P.host_ctl[SH_STOP]);
+          end = __shfl(stop, 0, 64) != 0;
+        }
+        if (end) {
           // no wave is active, so no delivery is in flight: tell the idle
           // waves through their mailboxes
           if (use_mbox)
@@ -404,18 +493,46 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
           continue;
         }
       }
+      // serving: wave 0 takes waiting requests while idle
+      // (only a wave still in the idle set may leave it to admit: one that is
+      // out of it has a delivery on the way, and must take that first, or a
+      // second sender could write its mailbox while it is read)
+      if (SERVE && gwave == 0 && (in_set || !use_mbox) && serve_waiting(P, lane)) {
+        bool left = true;
+        if (use_mbox) {
+          u64 old = 0;
+          if (lane == 0)
+            old = __hip_atomic_fetch_and(&P.idle_bits[gwave >> 6], ~(1ull << (gwave & 63)), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+          left = ((__shfl(old, 0, 64) >> (gwave & 63)) & 1) != 0;
+          in_set = false;  // either we left, or a sender took us out and a delivery is coming
+        }
+        if (left) {
+          if (lane == 0) __hip_atomic_fetch_add(&g->S, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          active = true;
+          serve_admit(P, L, lane, t_end);
+          __builtin_amdgcn_wave_barrier();
+          clamp_rings(top_c, bot_c, lane);
+          continue;
+        }
+      }
       if (!use_mbox || (idle_polls++ & (IDLE_CHECK - 1)) == 0) {
         u64 s = 0;
-        u32 err = 0;
+        u32 err = 0, stop = 0;
         if (lane == 0) {
           s = gld(&g->S);
           err = gld32(&g->err);
+          if (SERVE) stop = (u32)HVM_LD_SYS(&P.host_ctl[SH_STOP]);
         }
         s = __shfl(s, 0, 64);
         err = __shfl(err, 0, 64);
+        stop = __shfl(stop, 0, 64);
         S_cache = s;
-        if (err) break;
-        if (s == 0) break;
+        if (err) {
+          if (SERVE && lane == 0) HVM_ST_SYS(&P.host_ctl[SH_ERR], (u64)err);
+          break;
+        }
+        if (s == 0 && (!SERVE || stop)) break;
         if (limited && gld(&g->done) >= P.limit) break;
         if (wall_clock64() > t_end) {
           fail(L, ERR_TIMEOUT);
@@ -833,8 +950,9 @@ hipError_t launch_root_ports(const Params &P, u32 n, const u64 *roots, u64 *out,
   return hipGetLastError();
 }
 hipError_t launch_reduce(const Params &P, u32 blocks, hipStream_t st) {
-  if (P.gmap) hipLaunchKernelGGL(k_reduce<true>, dim3(blocks), dim3(WAVES_PER_BLOCK * 64), 0, st, P);
-  else hipLaunchKernelGGL(k_reduce<false>, dim3(blocks), dim3(WAVES_PER_BLOCK * 64), 0, st, P);
+  if (P.serve) hipLaunchKernelGGL((k_reduce<true, true>), dim3(blocks), dim3(WAVES_PER_BLOCK * 64), 0, st, P);
+  else if (P.gmap) hipLaunchKernelGGL((k_reduce<true, false>), dim3(blocks), dim3(WAVES_PER_BLOCK * 64), 0, st, P);
+  else hipLaunchKernelGGL((k_reduce<false, false>), dim3(blocks), dim3(WAVES_PER_BLOCK * 64), 0, st, P);
   return hipGetLastError();
 }
 hipError_t launch_readback(const Params &P, u64 root_cell, u64 q_head, u64 q_tail, u64 *stack, u64 stack_cap,
@@ -847,7 +965,7 @@ u32 reduce_waves_per_block() { return WAVES_PER_BLOCK; }
 u32 reduce_book_lds_words() { return BOOK_LDS_WORDS; }
 u32 reduce_lds_slots() { return LDS_SLOTS; }
 hipError_t reduce_occupancy(int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_reduce<false>, WAVES_PER_BLOCK * 64, 0);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_reduce<false, false>, WAVES_PER_BLOCK * 64, 0);
 }
 
 }  // namespace hvm

@@ -14,6 +14,10 @@
 #define HVM_CAS(p, e, v) \
   __hip_atomic_compare_exchange_strong((p), (e), (v), __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
 #define HVM_LDS_FAA(p, v) atomicAdd((p), (v))
+// Pinned host memory (serving rings): system-scope accesses.
+#define HVM_LD_SYS(p) __hip_atomic_load((p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)
+#define HVM_ST_SYS(p, v) __hip_atomic_store((p), (v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)
+#define HVM_FENCE_SYS() __builtin_amdgcn_fence(__ATOMIC_RELEASE, "")
 // A pointer known to address LDS (ds_* instructions, never flat ones).
 typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 #define HVM_TO_LDS(p) ((lds_u64 *)(p))
@@ -33,6 +37,9 @@ typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 #define HVM_FAA(p, v) __atomic_fetch_add((p), (v), __ATOMIC_RELAXED)
 #define HVM_CAS(p, e, v) __atomic_compare_exchange_n((p), (e), (v), false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)
 #define HVM_LDS_FAA(p, v) __atomic_fetch_add((p), (v), __ATOMIC_RELAXED)
+#define HVM_LD_SYS(p) __atomic_load_n((p), __ATOMIC_RELAXED)
+#define HVM_ST_SYS(p, v) __atomic_store_n((p), (v), __ATOMIC_RELAXED)
+#define HVM_FENCE_SYS() __atomic_thread_fence(__ATOMIC_RELEASE)
 typedef unsigned long long lds_u64;
 #define HVM_TO_LDS(p) ((lds_u64 *)(p))
 #define HVM_PUBLISH() ((void)0)
@@ -68,15 +75,18 @@ static const u32 BOOK_LDS_WORDS = 6144;
 // Every heap address is checked: a bad port (an engine bug) is redirected to
 // the sink words and recorded, so it can never touch memory outside the heap.
 // Records the first error of a launch; `site` says which check found it.
-NOINL void report_error(Globals *g, u32 code, u32 site = 0) {
+NOINL void report_error(Globals *g, u32 code, u32 site = 0, u64 what = 0) {
   u32 zero = 0;
-  if (HVM_CAS(&g->err, &zero, code)) HVM_ST(&g->err_info, site);
+  if (HVM_CAS(&g->err, &zero, code)) {
+    HVM_ST(&g->err_info, site);
+    HVM_ST(&g->result, what);  // the offending address or port, for the message
+  }
 }
 
 DEV u64 *hw(const Params &P, u64 addr) {
   u64 off = addr - HVM_HEAP_OFF;
   if (off >= P.heap_words * 8) {
-    report_error(P.g, ERR_BADPTR, BAD_HEAP_ADDR);
+    report_error(P.g, ERR_BADPTR, BAD_HEAP_ADDR, addr);
     off = DUMP_WORD * 8;
   }
   return P.heap + (off >> 3);
@@ -133,6 +143,12 @@ struct Lane {
   // sets it to a constant after lane_init, and the compiler drops the other
   // mode's code from the inlined rules)
   bool arenas;
+  // (serve: a serving launch, the arenas being request slots; also a
+  // compile-time constant in the kernel that serves)
+  bool serve;
+  u32 pushed;          // serving: redexes this task put in the wave's rings
+  u32 inl_n;           // serving: redexes without a heap port waiting to run in this task
+  u64 *inl;            // serving: their storage, SERVE_INL pairs
   u32 cur_net, a_net;
   u32 a_take;          // nodes of the lane's last take from a_net's arena
   u64 a_pos, a_end;
@@ -303,16 +319,18 @@ NOINL ArenaChunk arena_take(u64 *word, u64 base0, u64 len0, u64 max_refill, u32 
   return c;
 }
 
+// A port that reaches a heap node or cell (and so names its net).
+DEV bool reaches_heap(u64 p) { return ptag(p) >= T_OP || (ptag(p) == T_REF && addr_in_heap(paddr(p))); }
+
 // The net a redex belongs to: the arena of the node (or ExpandDef cell) it
 // reaches.  Two nilary ports allocate nothing, and a granule of the pool
 // whose owner this lane does not see yet names no net: the lane keeps its
 // net.
 DEV u32 net_of_port(const Params &P, u64 p, u32 dflt) {
-  const bool heap_ref = ptag(p) >= T_OP || (ptag(p) == T_REF && addr_in_heap(paddr(p)));
-  if (!heap_ref) return dflt;
+  if (!reaches_heap(p)) return dflt;
   const u64 g = ((paddr(p) - HVM_HEAP_OFF) >> 4) >> P.gran_shift;
   if (g >= P.gmap_len) {
-    report_error(P.g, ERR_BADPTR, BAD_HEAP_ADDR);
+    report_error(P.g, ERR_BADPTR, BAD_NET, p);
     return dflt;
   }
   const u32 net = HVM_LD(&P.gmap[g]);
@@ -471,6 +489,9 @@ NOINL void push_global(u64 *queue, u64 qmask, Globals *g, u64 a, u64 b) {
 }
 
 DEV void global_push1(Lane &L, u64 a, u64 b) {
+  // serving: a redex in the global ring can be taken and finished by another
+  // wave before this task ends, so it is counted for its net at once
+  if (L.serve) HVM_FAA(&L.P->pending[L.cur_net], 1u);
   push_global(L.P->queue, L.P->qmask, L.P->g, a, b);
   L.dirty = false;
 }
@@ -507,6 +528,18 @@ DEV void redux(Lane &L, u64 a, u64 b) {
     L.rw[RW_ERAS]++;
     return;
   }
+  if (L.serve) {
+    if (!reaches_heap(a) && !reaches_heap(b)) {  // no net to count it for: run it in this task
+      if (L.inl_n >= SERVE_INL) {
+        fail(L, ERR_QUEUE);
+        return;
+      }
+      L.inl[2 * L.inl_n] = a;
+      L.inl[2 * L.inl_n + 1] = b;
+      L.inl_n++;
+      return;
+    }
+  }
   if (L.bag) {
     u32 c = redex_class(a, b, L.book, L.P->ndefs);
     u32 idx = HVM_LDS_FAA(&L.bag_top[c], 1u);
@@ -514,6 +547,7 @@ DEV void redux(Lane &L, u64 a, u64 b) {
       u32 k = (ring_base(c) + (idx & (ring_cap(c) - 1))) * 2;
       L.bag[k] = a;
       L.bag[k + 1] = b;
+      if (L.serve) L.pushed++;  // counted when the task ends (a full ring's redex is counted at once, below)
       return;
     }
   }
@@ -1164,6 +1198,65 @@ DEV void interact(Lane &L, Interp &I, u32 &np, u64 a, u64 b) {
 // shared by every rule and instruction.
 static const u32 NO_DEF = 0xFFFFFFFFu;
 
+// Serving: slot `s`'s net is normal.  Its root port (following parked
+// ports) and tag go to the completion ring in host memory (the entry's
+// sequence word last), then the slot is reset and handed back for admission.
+DEV void serve_complete(Lane &L, u32 s) {
+  const Params &P = *L.P;
+  u64 p = var_port(HVM_LD(&P.slot_root[s]));
+  for (u32 hop = 0; hop < MAX_CHAIN && is_endpoint(p); hop++) {
+    const u64 v = ld(P, paddr(p));
+    if (v == P_NONE) break;
+    p = v;
+  }
+  u64 *const sv = P.serve_ctl;
+  const u64 idx = HVM_FAA(&sv[SV_DONE_TAIL], 1ull);
+  if (idx - HVM_LD_SYS(&P.host_ctl[SH_DONE_READ]) > P.done_mask) {  // the host fell behind by a whole ring
+    fail(L, ERR_QUEUE);
+    return;
+  }
+  u64 *e = P.done + (idx & P.done_mask) * 4;
+  HVM_ST_SYS(e + 1, HVM_LD(&P.slot_tag[s]));
+  HVM_ST_SYS(e + 2, p);
+  HVM_FENCE_SYS();
+  HVM_ST_SYS(e, idx + 1);
+  // the slot again: its arena restarts (lanes drop their chunks at every
+  // task, so no chunk of the old net is used again), then it is freed
+  HVM_ST(&P.arena_word[s], P.arena_base0[s] << 32);
+  publish();
+  const u64 pos = HVM_FAA(&sv[SV_FREE_TAIL], 1ull);
+  HVM_ST(&P.free_ring[pos % P.serve_slots], (u64)s + 1);
+  publish();
+  HVM_FAA(&sv[SV_FREE_COUNT], 1ull);
+}
+
+// Serving: the running task changed its net's count of redexes by `delta`
+// (the redexes it put in rings, less the one it consumed); the task that
+// brings the count to 0 completes the net.
+DEV void serve_account(Lane &L, u32 delta) {
+  if (delta == 0) return;
+  // always waited for, increments too: an increment still in flight when a
+  // lane of another iteration consumes the redex could let the count reach
+  // 0 early (measured: early completions, then a reused slot's stale redexes)
+  const u32 old = HVM_FAA(&L.P->pending[L.cur_net], delta);
+  if (old + delta == 0) serve_complete(L, L.cur_net);
+}
+
+// The instructions and link rounds of a task, then (serving) the redexes
+// without a heap port it made, each interacted here in turn.
+DEV void task_loop(Lane &L, Interp &I, u32 &np) {
+  for (;;) {
+    for (;;) {
+      if (I.left > 0) interp_run(L, I, np);
+      if (np > 0) flush_links(L, np);
+      if (I.left == 0) break;
+    }
+    if (!L.serve || L.inl_n == 0) break;
+    L.inl_n--;
+    interact(L, I, np, L.inl[2 * L.inl_n], L.inl[2 * L.inl_n + 1]);
+  }
+}
+
 DEV void run_task(Lane &L, bool has, u64 a, u64 b, u32 di, u64 t0) {
   Interp I;
   I.left = 0;
@@ -1174,16 +1267,50 @@ DEV void run_task(Lane &L, bool has, u64 a, u64 b, u32 di, u64 t0) {
   I.t_lds = false;
   u32 np = 0;
   if (has && L.arenas) L.cur_net = net_of_port(*L.P, a, net_of_port(*L.P, b, L.cur_net));
+  if (L.serve) {
+    L.pushed = 0;
+    L.a_net = NO_NET;  // a slot's arena restarts when its net completes
+  }
   if (has) interact(L, I, np, a, b);
   else if (di != NO_DEF) interp_begin(L, I, load_def(L, di), t0, load_cdef(L, di));
-  for (;;) {
-    if (I.left > 0) interp_run(L, I, np);
-    if (np > 0) flush_links(L, np);
-    if (I.left == 0) break;
-  }
+  task_loop(L, I, np);
   // nodes a template wrote may be reached only through the redexes it
   // pushed: retire their stores before any lane reads them
   publish_if_dirty(L);
+  if (L.serve && has) serve_account(L, L.pushed - 1u);
+}
+
+// Serving: boots request `tag` (definition di) into free slot s, as
+// Net::boot + the expansion of Net::normal (net.rs:30-32, 55-69) would.  The
+// slot's count is held at 1 while the boot runs, so that a redex the boot
+// sends to the global ring cannot complete the net early.
+DEV void serve_boot(Lane &L, u32 s, u32 di, u64 tag) {
+  const Params &P = *L.P;
+  L.cur_net = s;
+  L.a_net = NO_NET;
+  HVM_FAA(&P.pending[s], 1u);
+  HVM_ST(&P.slot_tag[s], tag);
+  const u64 r2 = alloc_node(L);
+  st2(P, r2, P_NONE, var_port(r2));
+  HVM_ST(&P.slot_root[s], r2);
+  const u64 r = alloc_var1(L);
+  L.rw[RW_DREF]++;
+  Interp I;
+  I.left = 0;
+  I.ip = nullptr;
+  I.ipl = 0;
+  I.Tl = nullptr;
+  I.Tg = nullptr;
+  I.t_lds = false;
+  u32 np = 0;
+  L.pushed = 0;
+  interp_begin(L, I, load_def(L, di), var_port(r), load_cdef(L, di));
+  task_loop(L, I, np);
+  push_link(L, np, var_port(r), expand_ref(r2 + 8));
+  task_loop(L, I, np);
+  publish_if_dirty(L);
+  publish();
+  serve_account(L, L.pushed - 1u);
 }
 
 // ---------------------------------------------------------------------------
@@ -1200,6 +1327,10 @@ DEV void lane_init(Lane &L, const Params &P, u32 gwave, u32 lane) {
   L.dirty = false;
   L.arenas = P.gmap != nullptr;
   L.cur_net = 0;
+  L.serve = false;  // set by the serving kernel
+  L.pushed = 0;
+  L.inl_n = 0;
+  L.inl = P.inl ? P.inl + ((u64)gwave * 64 + lane) * SERVE_INL * 2 : nullptr;
   L.a_net = NO_NET;
   L.a_take = 0;
   L.a_pos = 0;

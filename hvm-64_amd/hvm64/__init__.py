@@ -35,6 +35,7 @@ EXPORTS = [
     "hvm_gpu_boot_batch", "hvm_gpu_normal_batch", "hvm_gpu_root_ports", "hvm_gpu_readback_root",
     "hvm_gpu_boot_batch_noalloc", "hvm_gpu_batch_plan", "hvm_gpu_batch_run", "hvm_gpu_batch_tune",
     "hvm_gpu_batch_info", "hvm_gpu_batch_used", "hvm_gpu_alloc_count",
+    "hvm_gpu_serve_start", "hvm_gpu_serve_submit", "hvm_gpu_serve_poll", "hvm_gpu_serve_stop",
     "hvm_gpu_snapshot", "hvm_gpu_load_book_image", "hvm_gpu_set_compiled",
     "hvm_book_print", "hvm_book_label_sets", "hvm_net_apply_tree", "hvm_book_encode",
 ]
@@ -123,6 +124,11 @@ def load_library(path=None):
                                        ctypes.POINTER(u64)]
     lib.hvm_gpu_alloc_count.argtypes = [P, ctypes.POINTER(u64)]
     lib.hvm_gpu_batch_used.argtypes = [P, ctypes.POINTER(u64)]
+    lib.hvm_gpu_serve_start.argtypes = [P, ctypes.c_uint32, u64, u64]
+    lib.hvm_gpu_serve_submit.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(u64)]
+    lib.hvm_gpu_serve_poll.argtypes = [P, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.c_uint32,
+                                       ctypes.POINTER(ctypes.c_uint32)]
+    lib.hvm_gpu_serve_stop.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
     lib.hvm_gpu_root_ports.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(u64), ctypes.POINTER(u64)]
     lib.hvm_gpu_readback_root.argtypes = [P, u64, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
     lib.hvm_gpu_snapshot.argtypes = [P, ctypes.POINTER(NetImage), u64, u64, ctypes.POINTER(u64)]
@@ -505,6 +511,47 @@ class PlannedBatch:
     def last_kernel_ms(self):
         ms = ctypes.c_double()
         self.heap._check(self.heap._lib.hvm_gpu_last_kernel_ms(self.heap._h, ctypes.byref(ms)))
+        return ms.value
+
+
+class Serving:
+    """A serving launch on `heap` (hvm_gpu_serve_*): the running reduction
+    kernel admits submitted requests into `slots` arenas of `slot_nodes`
+    nodes and reports each request's root port when its net is normal."""
+
+    def __init__(self, heap, slots, slot_nodes, queue=1 << 16):
+        self.heap = heap
+        heap._check(heap._lib.hvm_gpu_serve_start(heap._h, slots, slot_nodes, queue))
+        self.running = True
+        self._cap = 1 << 16
+        self._tags = (ctypes.c_uint64 * self._cap)()
+        self._ports = (ctypes.c_uint64 * self._cap)()
+
+    def submit(self, defs, tags):
+        """Queues one request per (definition, tag); HvmError QUEUE when the
+        request ring lacks room (nothing queued)."""
+        n = len(defs)
+        ids = (ctypes.c_uint32 * n)(*[d.idx if hasattr(d, "idx") else int(d) for d in defs])
+        tg = (ctypes.c_uint64 * n)(*[int(t) for t in tags])
+        self.heap._check(self.heap._lib.hvm_gpu_serve_submit(self.heap._h, n, ids, tg))
+
+    def poll(self):
+        """[(tag, port)] of the requests finished since the last poll."""
+        out = []
+        while True:
+            n = ctypes.c_uint32()
+            self.heap._check(self.heap._lib.hvm_gpu_serve_poll(self.heap._h, self._tags, self._ports, self._cap,
+                                                               ctypes.byref(n)))
+            out.extend(zip(self._tags[:n.value], self._ports[:n.value]))
+            if n.value < self._cap:
+                return out
+
+    def stop(self):
+        """Ends the launch once every admitted request is finished; returns
+        the launch's duration in ms."""
+        ms = ctypes.c_double()
+        self.running = False
+        self.heap._check(self.heap._lib.hvm_gpu_serve_stop(self.heap._h, ctypes.byref(ms)))
         return ms.value
 
 

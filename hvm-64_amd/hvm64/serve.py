@@ -1,24 +1,25 @@
-"""Serving: continuous batching of independent nets on one device.
+"""Serving: requests admitted by the running reduction kernel.
 
-Requests arrive over time (seeded Poisson arrivals).  The device is split
-into `slots`: each slot is its own heap (Heap with a capped reduction grid)
-driven by its own host thread, so the kernels of different slots overlap on
-the device.  Whenever a slot is free it admits every request that has
-arrived and is still waiting, oldest first, while the admitted requests'
-node bounds fit the slot's arena bytes (and at most `max_batch` of them),
-and reduces them as one Batch.  A request's latency is the time from its
-arrival to the end of the batch that reduced it.
+Requests arrive over time (seeded Poisson arrivals).  One persistent
+reduction launch serves them all (hvm_gpu_serve_*, include/hvm64_amd.h): the
+heap is cut into `slots` arenas of `slot_nodes` nodes, a wave of the launch
+admits waiting requests into free slots between its iterations, and a
+request is finished when no redex of its net is left, which frees its slot
+for the next one.  A request's latency is the time from its arrival to the
+host's poll that saw it finished.
 
-The heaps, books and grids stay resident on the device for the whole run;
-only the batch's root cells move between host and device.
+The host loop submits each request when its arrival time has come and polls
+for finished ones; the book, the heap and the launch stay resident on the
+device for the whole run.
 """
 import math
 import random
-import threading
 import time
 from dataclasses import dataclass, field
 
-from . import Batch, Heap, Host, u24_port
+import ctypes
+
+from . import Heap, HvmError, Host, Serving, arena_sizes, u24_port
 from . import configs
 
 NODE_BYTES = 16
@@ -86,6 +87,7 @@ class ServeStats:
     slots: int
     per_slot_blocks: int
     slot_rewrites: list = field(default_factory=list)
+    kernel_ms: float = 0.0
 
 
 def percentile(xs, q):
@@ -97,108 +99,92 @@ def percentile(xs, q):
 
 
 class Server:
-    """`slots` heaps of `heap_bytes` each, every one with 1/slots of the
-    device's reduction grid, all holding the same book of mix variants."""
+    """One heap of `heap_bytes` holding the book of mix variants, served by
+    one persistent launch with `slots` request arenas."""
 
-    def __init__(self, slots=4, heap_bytes=16 << 30, max_batch=1024, book_seed=7, device=0, timeout_s=300):
-        probe = Heap(1 << 30, device=device)
-        full_blocks, _ = probe.grid()
-        probe.close()
-        self.blocks = max(1, full_blocks // slots)
-        self.slots = []
+    def __init__(self, slots=1024, heap_bytes=192 << 30, book_seed=7, device=0, timeout_s=300, slot_nodes=None,
+                 queue=1 << 16):
+        self.heap = Heap(heap_bytes, device=device)
+        self.heap.set_timeout(timeout_s)
         variants = [("tree_sum", d) for d in range(8, 17)] + [("bitonic", k) for k in range(6, 13)]
         book, self.table = configs.mix_book(variants, book_seed)
-        for _ in range(slots):
-            heap = Heap(heap_bytes, device=device, max_blocks=self.blocks)
-            heap.set_timeout(timeout_s)
-            host = Host(book, heap)
-            self.slots.append((heap, host))
-        self.heap_bytes = heap_bytes
-        self.max_batch = max_batch
+        self.host = Host(book, self.heap)
+        if slot_nodes is None:
+            # every variant alone, allocation counted; lanes take nodes in
+            # chunks (a serving lane drops its chunk at every task), so a
+            # slot holds twice the largest count (1.3 times ran dry)
+            defs = [self.host.defs[self.table[v][0]] for v in variants]
+            slot_nodes = 2 * max(arena_sizes(self.heap, defs, margin=1.0, extra=0)) + 4096
+        self.slot_nodes = slot_nodes
+        # as many slots as asked for, as far as the heap holds them (whole
+        # 16 KiB granules each on heaps up to 64 GiB, larger above)
+        gran = 1 << 10
+        while (heap_bytes // 16) // gran > (4 << 20):
+            gran <<= 1
+        per_slot = -(-slot_nodes // gran) * gran
+        self.n_slots = max(1, min(slots, (heap_bytes // 16 - gran) // per_slot))
+        self.queue = queue
+        self.blocks, _ = self.heap.grid()
 
     def close(self):
-        for heap, _ in self.slots:
-            heap.close()
-        self.slots = []
+        if self.heap is not None:
+            self.heap.close()
+            self.heap = None
 
     def prepare(self, requests):
         for r in requests:
             r.expected_port = u24_port(int(self.table[(r.kind, r.size)][1]))
-            r.bound_bytes = node_bound(r.kind, r.size) * NODE_BYTES
+            r.bound_bytes = self.slot_nodes * NODE_BYTES
 
     def run(self, requests):
         """Serves `requests` (arrival times relative to now); returns ServeStats."""
         self.prepare(requests)
         pending = sorted(requests, key=lambda r: r.arrival)
-        cond = threading.Condition()
-        state = {"next": 0, "batches": 0, "sizes": []}
-        slot_rewrites = [0] * len(self.slots)
-        errors = []
+        by_id = {r.id: r for r in requests}
+        defs = {v: self.host.defs[self.table[v][0]] for v in self.table}
+        srv = Serving(self.heap, self.n_slots, self.slot_nodes, self.queue)
         t0 = time.perf_counter()
-
-        def now():
-            return time.perf_counter() - t0
-
-        def admit():
-            # called with cond held: the longest prefix of arrived requests that fits
-            i = state["next"]
-            t = now()
-            taken, used = [], 0
-            while i < len(pending) and pending[i].arrival <= t and len(taken) < self.max_batch:
-                r = pending[i]
-                if taken and used + r.bound_bytes > self.heap_bytes:
-                    break
-                taken.append(r)
-                used += r.bound_bytes
-                i += 1
-            state["next"] = i
-            return taken
-
-        def worker(slot_id):
-            heap, host = self.slots[slot_id]
-            try:
-                while True:
-                    with cond:
-                        while True:
-                            if state["next"] >= len(pending):
-                                return
-                            batch = admit()
-                            if batch:
-                                state["batches"] += 1
-                                state["sizes"].append(len(batch))
-                                batch_id = state["batches"]
-                                break
-                            wait = pending[state["next"]].arrival - now()
-                            cond.wait(timeout=max(wait, 0.0005))
-                    start = now()
-                    b = Batch(heap)
-                    b.boot([host.defs[self.table[(r.kind, r.size)][0]] for r in batch])
-                    b.normal()
-                    ports = b.ports()
-                    end = now()
-                    slot_rewrites[slot_id] += b.rwts.total()
-                    for r, p in zip(batch, ports):
-                        r.start, r.done, r.port, r.slot, r.batch = start, end, p, slot_id, batch_id
-            except Exception as e:  # surfaced after the run
-                errors.append(e)
-                with cond:
-                    cond.notify_all()
-
-        threads = [threading.Thread(target=worker, args=(i,)) for i in range(len(self.slots))]
-        for th in threads:
-            th.start()
-        for th in threads:
-            th.join()
-        if errors:
-            raise errors[0]
-        wall = now()
+        nxt, finished, submits, sizes = 0, 0, 0, []
+        try:
+            while finished < len(requests):
+                t = time.perf_counter() - t0
+                k = nxt
+                while k < len(pending) and pending[k].arrival <= t:
+                    k += 1
+                if k > nxt:
+                    batch = pending[nxt:k]
+                    try:
+                        srv.submit([defs[(r.kind, r.size)] for r in batch], [r.id for r in batch])
+                        for r in batch:
+                            r.start = t
+                        sizes.append(len(batch))
+                        submits += 1
+                        nxt = k
+                    except HvmError as e:
+                        if e.kind != "QUEUE":
+                            raise
+                done = srv.poll()
+                t = time.perf_counter() - t0
+                for tag, port in done:
+                    r = by_id[tag]
+                    r.done, r.port = t, port
+                finished += len(done)
+                if not done and (nxt >= len(pending) or pending[nxt].arrival > t):
+                    time.sleep(0.0001)
+        finally:
+            kernel_ms = srv.stop()
+        wall = time.perf_counter() - t0
         done = [r for r in requests if r.done >= 0]
         lat = [r.latency * 1e3 for r in done]
-        rewrites = sum(slot_rewrites)
-        sizes = state["sizes"]
+        rewrites = self.heap_rewrites()
         return ServeStats(
             requests=len(requests), completed=len(done), all_ok=all(r.ok for r in requests),
             wall_s=wall, rewrites=rewrites, mips=rewrites / wall / 1e6 if wall > 0 else 0.0,
             p50_ms=percentile(lat, 50), p99_ms=percentile(lat, 99), max_ms=max(lat) if lat else float("nan"),
-            batches=len(sizes), mean_batch=sum(sizes) / len(sizes) if sizes else 0.0,
-            slots=len(self.slots), per_slot_blocks=self.blocks, slot_rewrites=slot_rewrites)
+            batches=submits, mean_batch=sum(sizes) / len(sizes) if sizes else 0.0,
+            slots=self.n_slots, per_slot_blocks=self.blocks, slot_rewrites=[rewrites], kernel_ms=kernel_ms)
+
+    def heap_rewrites(self):
+        out = (ctypes.c_uint64 * 5)()
+        self.heap._check(self.heap._lib.hvm_gpu_rewrites(self.heap._h, out))
+        return sum(int(x) for x in out)

@@ -205,6 +205,35 @@ int hvm_gpu_batch_info(hvm_gpu *h, hvm_batch_tuning *tuning, int *graph, uint64_
 int hvm_gpu_batch_used(hvm_gpu *h, uint64_t *used);
 int hvm_gpu_alloc_count(hvm_gpu *h, uint64_t *nodes);
 
+/* Serving: requests (a definition to boot, a caller's tag) admitted by the
+ * running reduction kernel itself.  No counterpart in the reference (one
+ * Net per Heap, reduced to normal form by Net::normal, net.rs:55-60, per
+ * request).
+ *
+ * hvm_gpu_serve_start: starts one persistent reduction launch on the
+ *   handle's heap, cut into `slots` arenas of `slot_nodes` 16-byte nodes
+ *   each (a request's net must fit its slot: HVM_E_OOM stops the launch
+ *   otherwise), with a request ring of at least `queue` entries in pinned
+ *   host memory.  The launch keeps running while the handle serves; every
+ *   other entry point is refused until hvm_gpu_serve_stop.
+ * hvm_gpu_serve_submit: queues n requests: boot definition def_ids[i] as
+ *   Net::boot + Net::normal would, answer with tags[i].  HVM_E_QUEUE when
+ *   the ring lacks room for all n (nothing is queued then).  One wave of the
+ *   launch polls the ring, and admits waiting requests into free slots (one
+ *   per lane, up to 64 at a time) between its own iterations.
+ * hvm_gpu_serve_poll: up to `cap` finished requests, in the order they
+ *   finished: tags[k] and ports[k], the port the request's root leads to
+ *   (a Num port for a numeric result; a tree's port is only valid until its
+ *   slot is reused).  A request is finished when no redex of its net is
+ *   left; its slot is then free for the next request.
+ * hvm_gpu_serve_stop: waits until every admitted request is finished and
+ *   the launch has ended; *ms (may be null) receives the launch's duration.
+ *   hvm_gpu_rewrites then reports the sums over every request served. */
+int hvm_gpu_serve_start(hvm_gpu *h, uint32_t slots, uint64_t slot_nodes, uint64_t queue);
+int hvm_gpu_serve_submit(hvm_gpu *h, uint32_t n, const uint32_t *def_ids, const uint64_t *tags);
+int hvm_gpu_serve_poll(hvm_gpu *h, uint64_t *tags, uint64_t *ports, uint32_t cap, uint32_t *n);
+int hvm_gpu_serve_stop(hvm_gpu *h, double *ms);
+
 /* ~ Net.rwts (runtime/src/runtime.rs:72-79): anni, comm, eras, dref, oper. */
 int hvm_gpu_rewrites(hvm_gpu *h, uint64_t out[5]);
 

@@ -37,21 +37,60 @@ def test_node_bound_covers_sequential_peak(oracle, kind, size):
 
 @pytest.mark.gpu
 def test_serving_results_alone_and_in_the_mix():
-    """Requests served in mixed batches on two concurrent slots give the
-    same result as each request reduced alone (integer results: exact)."""
+    """Requests admitted by the running kernel (hvm_gpu_serve_*) at 1,200
+    requests/s for one second, many in flight at once in one launch, give the
+    same result as each request reduced alone (integer results: exact), and
+    the launch's rewrite counts equal the sum of each request's counts."""
     import hvm64
-    print("case serving rate 400/s 0.5 s, 2 slots", flush=True)
-    srv = serve.Server(slots=2, heap_bytes=4 << 30, max_batch=256)
+    print("case serving 1200/s for 1 s, 128 slots", flush=True)
+    srv = serve.Server(slots=128, heap_bytes=64 << 30, timeout_s=60)
+    print("slots", srv.n_slots, "of", srv.slot_nodes, "nodes", flush=True)
+    assert srv.n_slots >= 64
     try:
-        reqs = serve.poisson_requests(400, 0.5, seed=5)
+        reqs = serve.poisson_requests(1200, 1.0, seed=5)
+        st = srv.run(reqs)
+        print("served", st.completed, "p50", st.p50_ms, "p99", st.p99_ms, "mips", st.mips, flush=True)
+        assert st.completed == len(reqs) and st.all_ok
+        assert st.p99_ms >= st.p50_ms > 0
+        served_total = st.rewrites
+        # alone: every variant that was served, each on a fresh net
+        alone = {}
+        for v in sorted({(r.kind, r.size) for r in reqs}):
+            net = hvm64.Net(srv.heap)
+            net.boot(srv.host.defs[srv.table[v][0]])
+            net.normal()
+            alone[v] = (net.rwts.total(), hvm64.u24_port(int(srv.host.readback(net))))
+        for r in reqs:
+            assert r.port == alone[(r.kind, r.size)][1] == r.expected_port
+        assert served_total == sum(alone[(r.kind, r.size)][0] for r in reqs)
+    finally:
+        srv.close()
+
+
+@pytest.mark.gpu
+def test_serving_refuses_other_calls_and_restarts():
+    """While serving, the handle's other entry points are refused; after
+    stop the handle reduces single nets again, and a second serving launch
+    on the same handle serves correctly."""
+    import hvm64
+    srv = serve.Server(slots=16, heap_bytes=8 << 30, timeout_s=60)
+    try:
+        s = hvm64.Serving(srv.heap, 16, srv.slot_nodes)
+        with pytest.raises(hvm64.HvmError) as e:
+            hvm64.Net(srv.heap)
+        assert e.value.kind == "ARG"
+        v = ("tree_sum", 9)
+        s.submit([srv.host.defs[srv.table[v][0]]] * 5, [1, 2, 3, 4, 5])
+        got = []
+        import time
+        t0 = time.time()
+        while len(got) < 5 and time.time() - t0 < 30:
+            got += s.poll()
+        s.stop()
+        assert sorted(t for t, _ in got) == [1, 2, 3, 4, 5]
+        assert all(p == hvm64.u24_port(int(srv.table[v][1])) for _, p in got)
+        reqs = serve.poisson_requests(300, 0.3, seed=6)
         st = srv.run(reqs)
         assert st.completed == len(reqs) and st.all_ok
-        assert st.batches >= 1 and st.p99_ms >= st.p50_ms > 0
-        heap, host = srv.slots[0]
-        for r in reqs[:6]:
-            net = hvm64.Batch(heap)
-            net.boot([host.defs[srv.table[(r.kind, r.size)][0]]])
-            net.normal()
-            assert net.ports() == [r.port] == [r.expected_port]
     finally:
         srv.close()
