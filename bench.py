@@ -398,6 +398,7 @@ def bench_serve(args):
         "rewrites": st.rewrites,
         "wall_s": round(st.wall_s, 3),
         "kernel_ms": round(st.kernel_ms, 3),
+        "sched": srv.last_sched,
     }
     print(json.dumps(out), flush=True)
     return 0 if st.all_ok else 1

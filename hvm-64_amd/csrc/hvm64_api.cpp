@@ -708,7 +708,9 @@ int hvm_gpu_serve_start(hvm_gpu *h, uint32_t slots, uint64_t slot_nodes, uint64_
     u32 gs = 10;
     while ((heap_nodes >> gs) > (4u << 20)) gs++;
     const u64 gmap_len = heap_nodes >> gs;
-    const u64 slot_g = (slot_nodes + (1ull << gs) - 1) >> gs;
+    // whole granules per sub-arena
+    const u64 sub_g = (slot_nodes + SERVE_SUB * (1ull << gs) - 1) / (SERVE_SUB * (1ull << gs));
+    const u64 slot_g = sub_g * SERVE_SUB;
     const u64 first = (RESERVED_WORDS / 2 + (1ull << gs) - 1) >> gs;  // granule 0: the fixed cells
     if (slot_g > 0xFFFFFFFFull || first + (u64)slots * slot_g > gmap_len)
       throw Error(HVM_E_OOM, "the slots' arenas exceed the heap");
@@ -720,7 +722,8 @@ int hvm_gpu_serve_start(hvm_gpu *h, uint32_t slots, uint64_t slot_nodes, uint64_
     sv.done_cap = dq;
     // device: pending | root | tag | free ring | counters | words | base0 | len | gmap | inl
     const u64 inl_words = (u64)h->waves * 64 * SERVE_INL * 2;
-    const u64 words = (slots + 1) / 2 + 4 * (u64)slots + SV_WORDS + (u64)slots + (slots + 1) / 2 + (gmap_len + 1) / 2 +
+    const u64 nsub = (u64)slots * SERVE_SUB;
+    const u64 words = (slots + 1) / 2 + 3 * (u64)slots + SV_WORDS + 2 * nsub + (nsub + 1) / 2 + (gmap_len + 1) / 2 +
                       inl_words + 64;
     hip_check(hipMalloc(&sv.dev, words * 8), "hipMalloc serving");
     u64 *w = sv.dev;
@@ -735,32 +738,35 @@ int hvm_gpu_serve_start(hvm_gpu *h, uint32_t slots, uint64_t slot_nodes, uint64_
     u64 *ctl = w;
     w += SV_WORDS;
     u64 *word = w;
-    w += slots;
+    w += nsub;
     u64 *base0 = w;
-    w += slots;
+    w += nsub;
     u32 *len = (u32 *)w;
-    w += (slots + 1) / 2;
+    w += (nsub + 1) / 2;
     u32 *gmap = (u32 *)w;
     w += (gmap_len + 1) / 2;
     u64 *inl = w;
     w += inl_words;
     u64 *nopool = w;  // the serving arenas do not grow: a pool and an overflow region of 0
-    std::vector<u64> h_word(slots), h_base0(slots), h_free(slots), h_ctl(SV_WORDS, 0);
-    std::vector<u32> h_len(slots, (u32)slot_g), h_gmap(gmap_len, NO_NET_HOST);
+    std::vector<u64> h_word(nsub), h_base0(nsub), h_free(slots), h_ctl(SV_WORDS, 0);
+    std::vector<u32> h_len(nsub, (u32)sub_g), h_gmap(gmap_len, NO_NET_HOST);
     for (u32 i = 0; i < slots; i++) {
-      h_base0[i] = first + (u64)i * slot_g;
-      h_word[i] = h_base0[i] << 32;
+      const u64 b = first + (u64)i * slot_g;
+      for (u32 k = 0; k < SERVE_SUB; k++) {
+        h_base0[(u64)i * SERVE_SUB + k] = b + k * sub_g;
+        h_word[(u64)i * SERVE_SUB + k] = (b + k * sub_g) << 32;
+      }
       h_free[i] = (u64)i + 1;
-      for (u64 g = h_base0[i]; g < h_base0[i] + slot_g; g++) h_gmap[g] = i;
+      for (u64 g = b; g < b + slot_g; g++) h_gmap[g] = i;
     }
     h_ctl[SV_FREE_TAIL] = slots;
     h_ctl[SV_FREE_COUNT] = slots;
     hip_check(hipMemset(pending, 0, (size_t)slots * 4), "init serving");
     hip_check(hipMemcpy(free_ring, h_free.data(), (size_t)slots * 8, hipMemcpyHostToDevice), "init serving");
     hip_check(hipMemcpy(ctl, h_ctl.data(), SV_WORDS * 8, hipMemcpyHostToDevice), "init serving");
-    hip_check(hipMemcpy(word, h_word.data(), (size_t)slots * 8, hipMemcpyHostToDevice), "init serving");
-    hip_check(hipMemcpy(base0, h_base0.data(), (size_t)slots * 8, hipMemcpyHostToDevice), "init serving");
-    hip_check(hipMemcpy(len, h_len.data(), (size_t)slots * 4, hipMemcpyHostToDevice), "init serving");
+    hip_check(hipMemcpy(word, h_word.data(), nsub * 8, hipMemcpyHostToDevice), "init serving");
+    hip_check(hipMemcpy(base0, h_base0.data(), nsub * 8, hipMemcpyHostToDevice), "init serving");
+    hip_check(hipMemcpy(len, h_len.data(), nsub * 4, hipMemcpyHostToDevice), "init serving");
     hip_check(hipMemcpy(gmap, h_gmap.data(), gmap_len * 4, hipMemcpyHostToDevice), "init serving");
     hip_check(hipMemset(nopool, 0, 64 * 8), "init serving");
     // pinned host memory: control words | requests | completions

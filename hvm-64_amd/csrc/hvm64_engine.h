@@ -208,6 +208,9 @@ enum : u32 { SH_REQ_TAIL = 0, SH_REQ_HEAD = 16, SH_DONE_READ = 32, SH_STOP = 48,
 // lane runs them at once, inside the task that made them (at most this many
 // pending per lane).
 static const u32 SERVE_INL = 32;
+// A serving slot's arena is this many sub-arenas (consecutive, whole
+// granules each), taken from by the lanes of matching index.
+static const u32 SERVE_SUB = 8;
 
 // Mailbox of a wave: a count word on its own 128-byte line, then up to
 // MBOX_PAIRS redex pairs handed to the wave by another wave.

@@ -421,7 +421,9 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
         break;
       }
     }
-    // ---- serving: wave 0 admits requests every fourth busy iteration
+    // ---- serving: the last task's count change (a net it completed), then
+    // wave 0 admits requests every fourth busy iteration
+    if (SERVE) serve_settle(L);
     if (SERVE && gwave == 0 && active && (++admit_tick & 3) == 0) {
       __builtin_amdgcn_wave_barrier();
       serve_admit(P, L, lane, t_end);

@@ -146,11 +146,15 @@ struct Lane {
   // (serve: a serving launch, the arenas being request slots; also a
   // compile-time constant in the kernel that serves)
   bool serve;
+  u32 lane_id;         // the lane's index in its wave
   u32 pushed;          // serving: redexes this task put in the wave's rings
   u32 inl_n;           // serving: redexes without a heap port waiting to run in this task
   u64 *inl;            // serving: their storage, SERVE_INL pairs
   u32 cur_net, a_net;
   u32 a_take;          // nodes of the lane's last take from a_net's arena
+  // serving: the last task's change to its net's count, whose old value is
+  // looked at only at the next loop pass (acct_delta 0: none)
+  u32 acct_old, acct_delta, acct_net;
   u64 a_pos, a_end;
   bool dirty;          // stores of new nodes/cells not yet retired
   // links deferred by the running definition: pair i at PL[2i * 64],
@@ -349,7 +353,12 @@ DEV void arena_room(Lane &L, u32 need) {
   t = t > P.arena_chunk ? P.arena_chunk : t;
   L.a_take = t;
   const u32 net = L.cur_net;
-  const ArenaChunk c = arena_take(&P.arena_word[net], P.arena_base0[net], P.arena_len[net], P.refill_granules,
+  // serving: a slot's arena is SERVE_SUB sub-arenas, each with its own word,
+  // a lane taking from the one of its lane index (a serving lane takes at
+  // nearly every task, and one word per slot was hammered by every lane on
+  // that slot)
+  const u32 wi = L.serve ? net * SERVE_SUB + (L.lane_id & (SERVE_SUB - 1)) : net;
+  const ArenaChunk c = arena_take(&P.arena_word[wi], P.arena_base0[wi], P.arena_len[wi], P.refill_granules,
                                   P.gran_shift, P.pool_next, P.pool_end, P.gmap, net, P.ov_next, P.ov_end,
                                   need > t ? need : t);
   L.a_net = L.cur_net;
@@ -1222,7 +1231,8 @@ DEV void serve_complete(Lane &L, u32 s) {
   HVM_ST_SYS(e, idx + 1);
   // the slot again: its arena restarts (lanes drop their chunks at every
   // task, so no chunk of the old net is used again), then it is freed
-  HVM_ST(&P.arena_word[s], P.arena_base0[s] << 32);
+  for (u32 k = 0; k < SERVE_SUB; k++)
+    HVM_ST(&P.arena_word[s * SERVE_SUB + k], P.arena_base0[s * SERVE_SUB + k] << 32);
   publish();
   const u64 pos = HVM_FAA(&sv[SV_FREE_TAIL], 1ull);
   HVM_ST(&P.free_ring[pos % P.serve_slots], (u64)s + 1);
@@ -1233,6 +1243,24 @@ DEV void serve_complete(Lane &L, u32 s) {
 // Serving: the running task changed its net's count of redexes by `delta`
 // (the redexes it put in rings, less the one it consumed); the task that
 // brings the count to 0 completes the net.
+// Serving, at the end of a task: the change is sent now, its old value
+// looked at in serve_settle at the wave's next loop pass, so the lane does not
+// wait for the atomic here.  Every path that shows this task's redexes to
+// another wave first retires the wave's memory operations (this one too),
+// and the wave pops its own rings only after serve_settle, so no consumer's
+// decrement can overtake it.
+DEV void serve_account_deferred(Lane &L, u32 delta) {
+  L.acct_delta = delta;
+  if (delta == 0) return;
+  L.acct_net = L.cur_net;
+  L.acct_old = HVM_FAA(&L.P->pending[L.cur_net], delta);
+}
+
+DEV void serve_settle(Lane &L) {
+  if (L.acct_delta != 0 && L.acct_old + L.acct_delta == 0) serve_complete(L, L.acct_net);
+  L.acct_delta = 0;
+}
+
 DEV void serve_account(Lane &L, u32 delta) {
   if (delta == 0) return;
   // always waited for, increments too: an increment still in flight when a
@@ -1277,7 +1305,7 @@ DEV void run_task(Lane &L, bool has, u64 a, u64 b, u32 di, u64 t0) {
   // nodes a template wrote may be reached only through the redexes it
   // pushed: retire their stores before any lane reads them
   publish_if_dirty(L);
-  if (L.serve && has) serve_account(L, L.pushed - 1u);
+  if (L.serve && has) serve_account_deferred(L, L.pushed - 1u);
 }
 
 // Serving: boots request `tag` (definition di) into free slot s, as
@@ -1328,11 +1356,13 @@ DEV void lane_init(Lane &L, const Params &P, u32 gwave, u32 lane) {
   L.arenas = P.gmap != nullptr;
   L.cur_net = 0;
   L.serve = false;  // set by the serving kernel
+  L.lane_id = lane;
   L.pushed = 0;
   L.inl_n = 0;
   L.inl = P.inl ? P.inl + ((u64)gwave * 64 + lane) * SERVE_INL * 2 : nullptr;
   L.a_net = NO_NET;
   L.a_take = 0;
+  L.acct_old = L.acct_delta = L.acct_net = 0;
   L.a_pos = 0;
   L.a_end = 0;
   L.T_lds = nullptr;

@@ -121,7 +121,7 @@ class Server:
         gran = 1 << 10
         while (heap_bytes // 16) // gran > (4 << 20):
             gran <<= 1
-        per_slot = -(-slot_nodes // gran) * gran
+        per_slot = -(-slot_nodes // (8 * gran)) * 8 * gran  # eight sub-arenas of whole granules
         self.n_slots = max(1, min(slots, (heap_bytes // 16 - gran) // per_slot))
         self.queue = queue
         self.blocks, _ = self.heap.grid()
@@ -177,12 +177,23 @@ class Server:
         done = [r for r in requests if r.done >= 0]
         lat = [r.latency * 1e3 for r in done]
         rewrites = self.heap_rewrites()
+        self.last_sched = self.sched_stats()
         return ServeStats(
             requests=len(requests), completed=len(done), all_ok=all(r.ok for r in requests),
             wall_s=wall, rewrites=rewrites, mips=rewrites / wall / 1e6 if wall > 0 else 0.0,
             p50_ms=percentile(lat, 50), p99_ms=percentile(lat, 99), max_ms=max(lat) if lat else float("nan"),
             batches=submits, mean_batch=sum(sizes) / len(sizes) if sizes else 0.0,
             slots=self.n_slots, per_slot_blocks=self.blocks, slot_rewrites=[rewrites], kernel_ms=kernel_ms)
+
+    def sched_stats(self):
+        """Scheduler statistics of the last serving launch (summed over waves)."""
+        out = (ctypes.c_uint64 * 20)()
+        self.heap._check(self.heap._lib.hvm_gpu_sched_stats(self.heap._h, out))
+        keys = ["busy_iters", "idle_iters", "busy_ticks", "spills", "spilled", "claims", "claimed",
+                "interact_ticks", "spill_ticks", "alloc_miss", "ov_alloc",
+                "anni_iters", "comm_iters", "call_iters", "other_iters",
+                "anni_ticks", "comm_ticks", "call_ticks", "other_ticks", "pops"]
+        return dict(zip(keys, [int(x) for x in out]))
 
     def heap_rewrites(self):
         out = (ctypes.c_uint64 * 5)()
