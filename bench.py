@@ -385,7 +385,7 @@ def bench_serve(args):
         "config": {"workload": f"Poisson arrivals at {args.rate:g}/s for {args.duration:g} s of tree-sum depth / "
                                f"bitonic k log-uniform over [8,16] / [6,12] (seed {args.seed})",
                    "rate": args.rate, "duration_s": args.duration, "slots": st.slots,
-                   "slot_nodes": srv.slot_nodes, "heap_gb": args.heap_gb, "grid_blocks": st.per_slot_blocks},
+                   "slot_nodes": srv.slot_nodes, "heap_gb": args.heap_gb, "grid_blocks": st.grid_blocks},
         "p50_ms": round(st.p50_ms, 2),
         "p99_ms": round(st.p99_ms, 2),
         "max_ms": round(st.max_ms, 2),
@@ -393,8 +393,8 @@ def bench_serve(args):
         "requests": st.requests,
         "completed": st.completed,
         "result_ok": st.all_ok,
-        "submits": st.batches,
-        "mean_submit": round(st.mean_batch, 1),
+        "submits": st.submits,
+        "mean_submit": round(st.mean_submit, 1),
         "rewrites": st.rewrites,
         "wall_s": round(st.wall_s, 3),
         "kernel_ms": round(st.kernel_ms, 3),

@@ -15,7 +15,7 @@ device for the whole run.
 import math
 import random
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import ctypes
 
@@ -23,16 +23,22 @@ from . import Heap, HvmError, Host, Serving, arena_sizes, u24_port
 from . import configs
 
 NODE_BYTES = 16
-# Parallel reduction keeps more of a net expanded at once than the
-# sequential normal() whose peak the bound starts from.
-PARALLEL_MARGIN = 16
+SUB_ARENAS = 8  # a slot's arena is cut in eight, taken from by lane index
 
 
-def node_bound(kind, size):
-    """Static node bound of one request: the oracle's sequential peak for
-    these programs is about 7 * 2**size nodes (tree-sum depth `size`,
-    bitonic 2**size keys), times PARALLEL_MARGIN."""
-    return (7 * (1 << size) + 256) * PARALLEL_MARGIN
+def slot_layout(heap_bytes, slot_nodes, slots):
+    """(slots that fit, nodes per slot, granule nodes) for a serving heap, as
+    hvm_gpu_serve_start lays it out: granules of 2^10 nodes, doubled until the
+    heap has at most 4M of them; the first granule holds the fixed cells;
+    every slot is eight sub-arenas of whole granules."""
+    heap_nodes = heap_bytes // NODE_BYTES
+    gran = 1 << 10
+    while heap_nodes // gran > (4 << 20):
+        gran <<= 1
+    sub = -(-slot_nodes // (SUB_ARENAS * gran))
+    per_slot = sub * SUB_ARENAS * gran
+    room = heap_nodes // gran - 1
+    return max(0, min(slots, room // (sub * SUB_ARENAS))), per_slot, gran
 
 
 @dataclass
@@ -42,12 +48,9 @@ class Request:
     kind: str
     size: int
     expected_port: int = 0
-    bound_bytes: int = 0
-    start: float = -1.0
-    done: float = -1.0
+    start: float = -1.0     # when it was submitted
+    done: float = -1.0      # when a poll saw it finished
     port: int = 0
-    slot: int = -1
-    batch: int = -1
 
     @property
     def latency(self):
@@ -82,11 +85,10 @@ class ServeStats:
     p50_ms: float
     p99_ms: float
     max_ms: float
-    batches: int
-    mean_batch: float
+    submits: int            # hvm_gpu_serve_submit calls
+    mean_submit: float      # requests per call
     slots: int
-    per_slot_blocks: int
-    slot_rewrites: list = field(default_factory=list)
+    grid_blocks: int
     kernel_ms: float = 0.0
 
 
@@ -116,13 +118,10 @@ class Server:
             defs = [self.host.defs[self.table[v][0]] for v in variants]
             slot_nodes = 2 * max(arena_sizes(self.heap, defs, margin=1.0, extra=0)) + 4096
         self.slot_nodes = slot_nodes
-        # as many slots as asked for, as far as the heap holds them (whole
-        # 16 KiB granules each on heaps up to 64 GiB, larger above)
-        gran = 1 << 10
-        while (heap_bytes // 16) // gran > (4 << 20):
-            gran <<= 1
-        per_slot = -(-slot_nodes // (8 * gran)) * 8 * gran  # eight sub-arenas of whole granules
-        self.n_slots = max(1, min(slots, (heap_bytes // 16 - gran) // per_slot))
+        # as many slots as asked for, as far as the heap holds them
+        self.n_slots, _, _ = slot_layout(heap_bytes, slot_nodes, slots)
+        if self.n_slots == 0:
+            raise HvmError(2, f"one slot of {slot_nodes} nodes exceeds the heap")  # HVM_E_OOM
         self.queue = queue
         self.blocks, _ = self.heap.grid()
 
@@ -134,7 +133,6 @@ class Server:
     def prepare(self, requests):
         for r in requests:
             r.expected_port = u24_port(int(self.table[(r.kind, r.size)][1]))
-            r.bound_bytes = self.slot_nodes * NODE_BYTES
 
     def run(self, requests):
         """Serves `requests` (arrival times relative to now); returns ServeStats."""
@@ -182,8 +180,8 @@ class Server:
             requests=len(requests), completed=len(done), all_ok=all(r.ok for r in requests),
             wall_s=wall, rewrites=rewrites, mips=rewrites / wall / 1e6 if wall > 0 else 0.0,
             p50_ms=percentile(lat, 50), p99_ms=percentile(lat, 99), max_ms=max(lat) if lat else float("nan"),
-            batches=submits, mean_batch=sum(sizes) / len(sizes) if sizes else 0.0,
-            slots=self.n_slots, per_slot_blocks=self.blocks, slot_rewrites=[rewrites], kernel_ms=kernel_ms)
+            submits=submits, mean_submit=sum(sizes) / len(sizes) if sizes else 0.0,
+            slots=self.n_slots, grid_blocks=self.blocks, kernel_ms=kernel_ms)
 
     def sched_stats(self):
         """Scheduler statistics of the last serving launch (summed over waves)."""

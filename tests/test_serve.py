@@ -26,13 +26,24 @@ def test_percentile():
     assert serve.percentile([7], 99) == 7
 
 
-@pytest.mark.parametrize("kind,size", [("tree_sum", 8), ("tree_sum", 12), ("bitonic", 6), ("bitonic", 9)])
-def test_node_bound_covers_sequential_peak(oracle, kind, size):
-    """The admission bound is at least the oracle's peak live nodes."""
-    text, expected = configs.variant(kind, size)
-    r = oracle.run(text, heap_words=1 << 25)
-    assert r["nf"] == expected
-    assert serve.node_bound(kind, size) >= serve.PARALLEL_MARGIN * r["peak_nodes"]
+@pytest.mark.parametrize("heap_gb,slot_nodes,slots", [(16, 1 << 20, 1024), (64, 28864928, 128),
+                                                     (192, 28864928, 1024), (192, 5000, 64)])
+def test_slot_layout_fits_the_heap(heap_gb, slot_nodes, slots):
+    """Every slot holds at least slot_nodes nodes in eight sub-arenas of whole
+    granules, the slots and the fixed-cell granule fit the heap, and one more
+    slot would not (when fewer than asked for fit)."""
+    heap_nodes = (heap_gb << 30) // 16
+    n, per_slot, gran = serve.slot_layout(heap_gb << 30, slot_nodes, slots)
+    assert heap_nodes // gran <= 4 << 20 and (gran == 1024 or heap_nodes // (gran // 2) > 4 << 20)
+    assert per_slot >= slot_nodes and per_slot % (8 * gran) == 0 and per_slot - slot_nodes < 8 * gran
+    assert 0 < n <= slots
+    assert gran + n * per_slot <= heap_nodes
+    if n < slots:
+        assert gran + (n + 1) * per_slot > (heap_nodes // gran) * gran
+
+
+def test_slot_layout_rejects_a_slot_beyond_the_heap():
+    assert serve.slot_layout(1 << 30, 1 << 30, 4)[0] == 0
 
 
 @pytest.mark.gpu
