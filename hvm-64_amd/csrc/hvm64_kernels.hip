@@ -389,6 +389,11 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   const u64 t_end = P.timeout_ticks ? wall_clock64() + P.timeout_ticks : ~0ull;
   u32 backoff = 1 + (gwave & 7);
   u32 busy = 0;                // busy iterations since the last control read
+  // The lanes that run an iteration's redexes start where the last
+  // iteration's left off (wave-uniform), so every lane's allocation slice
+  // fills at the same pace (with lane i taking the i-th redex, low lanes ran
+  // nearly every task and lapped their slices long before the others).
+  u32 rot = 0;
   u64 S_cache = 0;             // last value of S seen by this wave
   u64 S_next = 0;              // lane 0: control words read at iteration end
   u32 err_next = 0;
@@ -647,13 +652,15 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     // ---- pop (LIFO) and interact
     const u64 t_it = wall_clock64();
     st[ST_BUSY_IT]++;
-    bool has = lane < take;
+    const u32 nth = (lane - rot) & 63;  // this lane runs the nth-newest redex
+    bool has = nth < take;
     u64 ra = 0, rb = 0;
     if (has) {
-      u32 i = (ring_base(c) + ((top - 1 - lane) & (ring_cap(c) - 1))) * 2;
+      u32 i = (ring_base(c) + ((top - 1 - nth) & (ring_cap(c) - 1))) * 2;
       ra = bag[i];
       rb = bag[i + 1];
     }
+    rot = (rot + take) & 63;
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) top_c[c] = top - take;
     __builtin_amdgcn_wave_barrier();
