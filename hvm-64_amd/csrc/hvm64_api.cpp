@@ -929,20 +929,20 @@ int hvm_gpu_open_grid(int device, uint64_t heap_bytes, uint32_t max_blocks, hvm_
     if (words < min_words) words = min_words;
     h->heap_words = words;
     hip_check(hipMalloc(&h->heap, words * 8), "hipMalloc heap");
-    // Node layout: reserved words, one ring of 2^k nodes per wave (k as large
-    // as fits in 7/8 of the heap; each lane allocates every 64th node of its
-    // wave's ring), then the shared overflow ring (2^j nodes).
+    // Node layout: reserved words, one ring per wave (each lane allocates
+    // every 64th node of its wave's ring), then the shared overflow ring of
+    // 2^j nodes, the largest power of two within a sixteenth of the heap.
+    // The rings share the rest evenly (a multiple of 64 nodes each): a lane
+    // laps its slice only once it has allocated that many nodes.
     u64 nodes = (words - RESERVED_WORDS) / 2;
-    u64 per_wave = 1;
-    while (per_wave * 2 * h->waves <= nodes / 8 * 7) per_wave *= 2;
+    u64 ov = 64;
+    while (ov * 2 <= nodes / 16) ov *= 2;
+    if (nodes < ov) throw std::runtime_error("heap too small for the grid");
+    const u64 per_wave = (nodes - ov) / h->waves / 64 * 64;
     if (per_wave < 64) throw std::runtime_error("heap too small for the grid");
     h->ring_node0 = RESERVED_WORDS / 2;
     h->wave_nodes = per_wave;
-    u64 rest = nodes - per_wave * h->waves;
-    u64 ov = 1;
-    while (ov * 2 <= rest) ov *= 2;
     h->ov_node0 = h->ring_node0 + per_wave * h->waves;
-    if (ov < 64) throw std::runtime_error("heap too small for the grid");
     h->ov_nodes = ov;
     hip_check(hipMalloc(&h->alloc_pos, (size_t)h->waves * 64 * sizeof(u64)), "hipMalloc rings");
     h->idle_words = (h->waves + 63) / 64;

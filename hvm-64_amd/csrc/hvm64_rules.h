@@ -182,9 +182,9 @@ DEV void fail(Lane &L, u32 code, u32 site = 0) { report_error(L.P->g, code, site
 
 DEV u64 dump_addr() { return HVM_HEAP_OFF + DUMP_WORD * 8; }
 
-// Ring allocation.  Every wave owns a ring of 2^k nodes and every lane a
-// 1/64 slice of it (every 64th node, so lanes allocating together write
-// adjacent nodes).  A slice's position counts its allocations since the heap
+// Ring allocation.  Every wave owns a ring (a multiple of 64 nodes) and
+// every lane a 1/64 slice of it (every 64th node, so lanes allocating
+// together write adjacent nodes).  A slice's position counts its allocations since the heap
 // was reset, so on the first lap every slot is known to be free and is taken
 // without reading it.  Later laps take the next slot whose two words are
 // FREE (frees write FREE in place, from any wave).  When SCAN_MISSES slots in
@@ -213,7 +213,7 @@ NOINL AllocCursor alloc_slow(u64 *heap, Globals *g, u64 wave_nodes, u64 ov_node0
   c.err = 0;
   if (c.slice_rest == 0) {
     for (u32 miss = 0; miss < SCAN_MISSES; miss++) {
-      u64 node = sub_node0 + (c.apos & (n - 1)) * 64;
+      u64 node = sub_node0 + (c.apos % n) * 64;
       c.apos++;
       u64 w0 = HVM_LD(heap + 2 * node), w1 = HVM_LD(heap + 2 * node + 1);
       if (w0 == P_FREE && w1 == P_FREE) {
