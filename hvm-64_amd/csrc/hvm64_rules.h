@@ -137,6 +137,9 @@ struct Lane {
   u32 slice_rest;      // allocations left to take from the overflow ring
   u32 ov_left;         // nodes left in the current overflow run
   u64 ov_cur;          // next overflow-ring position of that run
+  // free cache: nodes this lane consumed, newest first (fc0 .. fc{fcn-1})
+  u64 fc0, fc1, fc2, fc3;
+  u32 fcn;
   // arenas: the net of the running task, and the lane's chunk of an arena
   // (nodes [a_pos, a_end) of net a_net; a_net = NO_NET: none)
   // (arenas: allocation from per-net arenas; a kernel that knows its mode
@@ -193,6 +196,7 @@ DEV u64 dump_addr() { return HVM_HEAP_OFF + DUMP_WORD * 8; }
 // nodes (one atomic per run; a run in the overflow ring's first lap is free
 // without reads, later laps check every slot).
 static const u32 SCAN_MISSES = 8;
+static const u32 FREE_CACHE = 4;  // nodes a lane keeps for reuse (registers)
 static const u32 SLICE_REST = 256;
 static const u32 OV_RUN = 64;
 
@@ -375,6 +379,14 @@ DEV u64 alloc_node(Lane &L) {
     arena_room(L, 1);
     return L.a_pos < L.a_end ? HVM_HEAP_OFF + (L.a_pos++) * 16 : dump_addr();
   }
+  if (L.fcn) {
+    const u64 a = L.fc0;
+    L.fc0 = L.fc1;
+    L.fc1 = L.fc2;
+    L.fc2 = L.fc3;
+    L.fcn--;
+    return a;
+  }
   const u64 n = L.P->wave_nodes / 64;
   if (L.apos < n) return HVM_HEAP_OFF + (L.sub_node0 + (L.apos++) * 64) * 16;
   AllocCursor c;
@@ -458,20 +470,45 @@ DEV u64 new_node(Lane &L, u32 tag, u32 lab, u64 p1, u64 p2) {
   return mkport(tag, lab, n);
 }
 
-// Consume a binary node: read both aux words, then free the node.
+// A node this lane has consumed (its words already read).  Per-net arenas
+// never reuse a node, so it is simply dropped; with the allocation rings it
+// goes to the lane's free cache, from which the lane's next allocations are
+// taken, or, when the cache is full, it is marked FREE for the ring scan.  A
+// cached node is never marked: the scan skips it as live until it is reused.
+DEV void drop_node(Lane &L, u64 *w) {
+  if (L.arenas) return;
+  const u64 off = (u64)(w - L.P->heap);
+  if (L.fcn < FREE_CACHE && off >= RESERVED_WORDS) {
+    L.fc3 = L.fc2;
+    L.fc2 = L.fc1;
+    L.fc1 = L.fc0;
+    L.fc0 = HVM_HEAP_OFF + off * 8;
+    L.fcn++;
+    return;
+  }
+  st16(w, P_FREE, P_FREE);
+}
+
+// Consume a binary node: read both aux words, then drop the node.
 DEV void take(Lane &L, u64 p, u64 &x, u64 &y) {
   u64 a = paddr(p);
   u64 *w = hw(*L.P, a & ~15ull);
   ld16(w, x, y);
-  st16(w, P_FREE, P_FREE);
+  drop_node(L, w);
 }
 
 // take() of two nodes, their loads in flight together.
 DEV void take2(Lane &L, u64 p, u64 q, u64 &p1, u64 &p2, u64 &q1, u64 &q2) {
   u64 *wp = hw(*L.P, paddr(p) & ~15ull), *wq = hw(*L.P, paddr(q) & ~15ull);
   ld16x2(wp, wq, p1, p2, q1, q2);
-  st16(wp, P_FREE, P_FREE);
-  st16(wq, P_FREE, P_FREE);
+  drop_node(L, wp);
+  drop_node(L, wq);
+}
+
+// A var cell both of whose sides have arrived: marked FREE for the ring scan
+// (arenas never reuse it).
+DEV void free_cell(Lane &L, u64 c) {
+  if (!L.arenas) st(*L.P, c, P_FREE);
 }
 
 // Retire this lane's stores before a port to new memory becomes visible.
@@ -628,7 +665,7 @@ DEV void flush_links(Lane &L, u32 &np) {
 #pragma unroll
     for (u32 j = 0; j < 4; j++) {
       if (o[j] != P_NONE) {  // second arrival: free the cell, link on
-        st(P, c[j], P_FREE);
+        free_cell(L, c[j]);
         PL[(u64)(2 * np) * 64] = o[j];
         PL[(u64)(2 * np + 1) * 64] = b[j];
         np++;
@@ -650,7 +687,7 @@ DEV u64 resolve(Lane &L, u64 p) {
     u64 c = paddr(p);
     u64 v = ld(P, c);
     if (v == P_NONE) break;
-    st(P, c, P_FREE);
+    free_cell(L, c);
     p = v;
   }
   return p;
@@ -797,7 +834,7 @@ DEV void st2_new(const Params &P, u64 addr, u64 w0, u64 w1) {
 
 // Two fresh nodes: consecutive slice positions on the first lap.
 DEV void alloc2(Lane &L, u64 &n0, u64 &n1) {
-  if (!L.arenas && L.apos + 2 <= L.P->wave_nodes / 64) {
+  if (!L.arenas && L.fcn == 0 && L.apos + 2 <= L.P->wave_nodes / 64) {
     n0 = HVM_HEAP_OFF + (L.sub_node0 + L.apos * 64) * 16;
     n1 = n0 + 64 * 16;
     L.apos += 2;
@@ -1352,6 +1389,8 @@ DEV void lane_init(Lane &L, const Params &P, u32 gwave, u32 lane) {
   L.slice_rest = 0;
   L.ov_left = 0;
   L.ov_cur = 0;
+  L.fcn = 0;
+  L.fc0 = L.fc1 = L.fc2 = L.fc3 = 0;
   L.dirty = false;
   L.arenas = P.gmap != nullptr;
   L.cur_net = 0;
