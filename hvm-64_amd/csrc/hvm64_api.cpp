@@ -891,9 +891,10 @@ int hvm_gpu_alloc_count(hvm_gpu *h, uint64_t *nodes) {
     hip_check(hipStreamSynchronize(h->stream), "sync");
     hip_check(hipMemcpy(pos.data(), h->alloc_pos, pos.size() * 8, hipMemcpyDeviceToHost), "read rings");
     // ring positions advanced, less the slots found in use, plus the
-    // overflow ring's allocations
+    // overflow ring's allocations and the nodes lanes reused from their free
+    // caches (an arena reuses nothing, so it needs all of them)
     hip_check(hipMemcpy(&h->last, h->g, sizeof(Globals), hipMemcpyDeviceToHost), "read globals");
-    u64 sum = h->last.rw[RW_OV_ALLOC];
+    u64 sum = h->last.rw[RW_OV_ALLOC] + h->last.rw[RW_REUSE];
     for (u64 v : pos) sum += v;
     *nodes = sum - std::min(sum, (u64)h->last.rw[RW_ALLOC_MISS]);
   });

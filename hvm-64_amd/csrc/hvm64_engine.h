@@ -71,6 +71,7 @@ enum : u32 {
   RW_ANNI = 0, RW_COMM, RW_ERAS, RW_DREF, RW_OPER, SC_OP_ERA, SC_OPNUM_ERA, SC_SWITCH_ERA, RW_POPS,
   RW_ALLOC_MISS,  // ring slots found in use on a later lap
   RW_OV_ALLOC,    // allocations served by the overflow ring
+  RW_REUSE,       // allocations served by a lane's free cache
   RW_SLOTS
 };
 

@@ -385,6 +385,7 @@ DEV u64 alloc_node(Lane &L) {
     L.fc1 = L.fc2;
     L.fc2 = L.fc3;
     L.fcn--;
+    L.rw[RW_REUSE]++;
     return a;
   }
   const u64 n = L.P->wave_nodes / 64;

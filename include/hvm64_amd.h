@@ -187,8 +187,9 @@ int hvm_gpu_readback_root(hvm_gpu *h, uint64_t root, char *buf, uint64_t cap, ui
  *   this includes the unused ends of chunks): the measure that sizes the
  *   next plan's arenas.
  * hvm_gpu_alloc_count: nodes allocated since the last reset by the per-lane
- *   allocation rings (a single net or an unplanned batch): the measure that
- *   sizes arenas.
+ *   allocation rings (a single net or an unplanned batch), counting a node a
+ *   lane reused from its free cache as a new allocation (an arena reuses
+ *   nothing): the measure that sizes arenas.
  * No counterpart in the reference (whose nets each own a heap). */
 typedef struct {
   uint32_t blocks;       /* reduce workgroups (0: the handle's grid) */
