@@ -40,6 +40,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <set>
 #include <memory>
 #include <pthread.h>
 #include <sstream>
@@ -742,7 +743,7 @@ struct InterpretedDef {
 struct Net;
 struct alignas(32) Def {  // def.rs:86-95
   LabSet labs;
-  int kind = 0;  // 0 interpreted, 1 expand
+  int kind = 0;  // 0 interpreted, 1 expand, 2 inert (pre_reduce.rs:99-116)
   InterpretedDef data;
   Port out{0};  // ExpandDef payload (net.rs:72-74)
   // A compiled definition (AsDef::call of the structs src/compile.rs:110-161
@@ -817,6 +818,9 @@ struct Net {
   Rewrites rwts;
   RedexQueue redexes;
   Shortcuts shortcuts;
+  // Redexes an inert definition was called on (InertDef::call,
+  // transform/src/pre_reduce.rs:108-115), in call order.
+  std::vector<std::pair<Port, Port>> captured;
   std::vector<Trg> trgs;
   Wire root{0};
   size_t tid = 0, tids = 1;
@@ -1084,6 +1088,7 @@ struct Net {
     if (trg.tag() == Ctr && !def->labs.has(trg.lab())) return comm02(port, trg);
     rwts.dref++;
     if (def->kind == 1) return expand_call(def, trg);
+    if (def->kind == 2) return captured.emplace_back(port, trg), void();
     if (def->compiled) return def->compiled(*this, trg);
     interpret(def->data, Trg::port(trg));
   }
@@ -1394,6 +1399,9 @@ struct LabelCalc {  // calc_labels.rs:78-194
     LabSet labs;
   };
   std::map<std::string, State> labels;
+  // calc_labels.rs:183-190: a reference to a name that is not in the book
+  // takes its label set from `lookup` (host.rs:53-59), or is an error.
+  std::function<LabSet(const std::string &)> lookup;
   explicit LabelCalc(const Book &b) : book(b) {}
   static const size_t NONE = SIZE_MAX;  // "depth = None" (second pass)
 
@@ -1437,7 +1445,8 @@ struct LabelCalc {  // calc_labels.rs:78-194
     if (t.kind == Tree::CtrK && out) out->add(t.lab);
     if (t.kind == Tree::RefK) {
       if (book.count(t.name)) return visit_def(t.name, depth == NONE ? NONE : depth + 1, out);
-      throw HvmError(E_NOREF, "Found reference @" + t.name + ", which is not in the book!");
+      if (!lookup) throw HvmError(E_NOREF, "Found reference @" + t.name + ", which is not in the book!");
+      if (out) out->unite(lookup(t.name));
     }
     size_t m = SIZE_MAX;
     if (t.a) m = std::min(m, visit_tree(*t.a, depth, out));
@@ -1456,8 +1465,26 @@ struct Host {
   std::map<std::string, std::unique_ptr<Def>> defs;
   std::unordered_map<uint64_t, std::string> back;
 
-  void insert_book(const Book &book) {  // host.rs:41-73
-    auto sets = LabelCalc(book).run();
+  // host.rs:41-73 with the default of pre_reduce.rs:44-45: every name the
+  // book refers to but does not define becomes an inert definition
+  // (LabSet::ALL) that captures the redexes it is called on.
+  void insert_book_with_inert_default(const Book &book) {
+    LabelCalc calc(book);
+    calc.lookup = [&](const std::string &name) {
+      auto it = defs.find(name);
+      if (it == defs.end()) {
+        auto d = std::make_unique<Def>();
+        d->labs = LabSet::ALL();
+        d->kind = 2;
+        back[new_ref(d.get()).addr()] = name;
+        it = defs.emplace(name, std::move(d)).first;
+      }
+      return it->second->labs;
+    };
+    insert_sets(book, calc.run());
+  }
+  void insert_book(const Book &book) { insert_sets(book, LabelCalc(book).run()); }
+  void insert_sets(const Book &book, const std::map<std::string, LabSet> &sets) {  // host.rs:41-73
     for (auto &kv : sets) {
       auto d = std::make_unique<Def>();
       d->labs = kv.second;
@@ -1788,6 +1815,78 @@ static RunResult run_book(const std::string &text, const std::string &entry, siz
 }
 
 // ---------------------------------------------------------------------------
+// Book::pre_reduce (transform/src/pre_reduce.rs:38-164): every definition not
+// skipped is booted on a fresh net (all nets share one heap, as there) and
+// reduced for at most `max_rwts` interactions, its dependencies first
+// (references in tree order: root, then each redex's two sides; a definition
+// met again while its own dependencies are visited is left as it is); the
+// redexes an inert definition captured go back into the net; the read-back
+// net replaces the definition's instructions (label sets stay those of the
+// original book) and, in the returned book, its net.  Missing definitions
+// are inert (pre_reduce.rs:44-45).
+// ---------------------------------------------------------------------------
+struct PreReduceResult {
+  std::string book;
+  Rewrites rwts;
+  std::vector<std::string> not_normal;
+};
+
+static PreReduceResult pre_reduce_book(const std::string &text, const std::function<bool(const std::string &)> &skip,
+                                       size_t heap_words, uint64_t max_rwts) {
+  PreReduceResult res;
+  with_big_stack([&] {
+    Book book = parse_book(text);
+    Host host;
+    host.insert_book_with_inert_default(book);
+    Heap heap(heap_words);
+    std::map<std::string, int> seen;  // 1: being visited (Cycled), 2: reduced
+    std::map<std::string, AstNet> reduced;
+    std::function<void(const std::string &)> pre;
+    std::function<void(const Tree &)> visit_tree = [&](const Tree &t) {  // pre_reduce.rs:120-128
+      if (t.kind == Tree::RefK) pre(t.name);
+      if (t.a) visit_tree(*t.a);
+      if (t.b) visit_tree(*t.b);
+    };
+    pre = [&](const std::string &name) {  // pre_reduce.rs:136-163
+      if (seen.count(name) || skip(name) || !book.count(name)) return;
+      seen[name] = 1;
+      const AstNet &src = book.at(name);
+      visit_tree(src.root);
+      for (auto &r : src.redexes) {
+        visit_tree(r.first);
+        visit_tree(r.second);
+      }
+      Net rt(&heap);
+      rt.boot(host.defs.at(name).get());
+      size_t count = 0;
+      const bool normal = rt.reduce((size_t)std::min<uint64_t>(max_rwts, SIZE_MAX), count);
+      res.rwts.anni += rt.rwts.anni;
+      res.rwts.comm += rt.rwts.comm;
+      res.rwts.eras += rt.rwts.eras;
+      res.rwts.dref += rt.rwts.dref;
+      res.rwts.oper += rt.rwts.oper;
+      for (auto &c : rt.captured) rt.redux(c.first, c.second);
+      rt.captured.clear();
+      AstNet net = host.readback(rt);
+      host.defs.at(name)->data = host.encode_def(net);
+      if (!normal) res.not_normal.push_back(name);
+      reduced[name] = std::move(net);
+      seen[name] = 2;
+    };
+    for (auto &kv : book) pre(kv.first);
+    bool first = true;
+    for (auto &kv : book) {  // ast.rs:221-232
+      if (!first) res.book += "\n\n";
+      first = false;
+      auto it = reduced.find(kv.first);
+      res.book += "@" + kv.first + " = " + print_net(it != reduced.end() ? it->second : kv.second);
+    }
+    std::free(heap.nodes);
+  });
+  return res;
+}
+
+// ---------------------------------------------------------------------------
 // Compiled definitions (src/compile.rs:31-161): one C++ function per
 // definition, the instruction list written out as straight-line calls of
 // the interpreter's own do_* methods, as the reference's compiler writes
@@ -2036,6 +2135,30 @@ int oracle_apply_tree(const char *fun, const char *const *args, int nargs, uint6
 }
 
 // Book -> canonical text (parse + Display), for parser round-trip checks.
+// Book::pre_reduce on `book`: `names` (n of them) are the definitions to
+// skip, or, with only != 0, the only ones to reduce.  Writes the new book's
+// text and the pre-reduction's five counts (anni comm eras dref oper).
+int oracle_pre_reduce(const char *book, const char *const *names, uint32_t n, int only, uint64_t max_rwts,
+                      uint64_t heap_words, char *out, uint64_t cap, uint64_t *len, uint64_t rwts[5]) {
+  try {
+    std::set<std::string> listed;
+    for (uint32_t i = 0; i < n; i++) listed.insert(names[i]);
+    auto skip = [&](const std::string &x) { return only ? listed.count(x) == 0 : listed.count(x) != 0; };
+    PreReduceResult r = pre_reduce_book(book, skip, heap_words, max_rwts);
+    if (rwts) {
+      rwts[0] = r.rwts.anni;
+      rwts[1] = r.rwts.comm;
+      rwts[2] = r.rwts.eras;
+      rwts[3] = r.rwts.dref;
+      rwts[4] = r.rwts.oper;
+    }
+    return copy_out(r.book, out, cap, len);
+  } catch (HvmError &e) {
+    g_err = e.what();
+    return 100 + e.code;
+  }
+}
+
 int oracle_print_book(const char *book, char *out, uint64_t cap, uint64_t *len) {
   try {
     std::string text;

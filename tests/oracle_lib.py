@@ -39,6 +39,9 @@ class Oracle:
         lib.oracle_parse_num.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32)]
         lib.oracle_create_var.argtypes = [u64, ctypes.c_char_p, u64]
         lib.oracle_compile_book.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, u64, ctypes.POINTER(u64)]
+        lib.oracle_pre_reduce.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), ctypes.c_uint32,
+                                          ctypes.c_int, u64, u64, ctypes.c_char_p, u64, ctypes.POINTER(u64),
+                                          ctypes.POINTER(u64)]
 
     def _err(self, rc):
         raise OracleError(f"oracle error {rc}: {self.lib.oracle_last_error().decode()}")
@@ -92,6 +95,28 @@ class Oracle:
         if rc:
             self._err(rc)
         return buf.value.decode()
+
+    def pre_reduce(self, book, skip=(), only=None, max_rwts=(1 << 64) - 1, heap_words=1 << 26):
+        """Book::pre_reduce (transform/src/pre_reduce.rs:38-75): reduces every
+        definition except those in `skip` (or, with `only`, just those).
+        Returns (book_text, rwts dict)."""
+        names = list(only if only is not None else skip)
+        arr = (ctypes.c_char_p * max(len(names), 1))(*[x.encode() for x in names])
+        cap = 1 << 22
+        while True:
+            buf = ctypes.create_string_buffer(cap)
+            n = ctypes.c_uint64()
+            c = (ctypes.c_uint64 * 5)()
+            rc = self.lib.oracle_pre_reduce(book.encode(), arr, len(names), int(only is not None), max_rwts, heap_words,
+                                            buf, cap, ctypes.byref(n), c)
+            if rc == 7 and n.value + 1 > cap:
+                cap = n.value + 1
+                continue
+            if rc:
+                self._err(rc)
+            rw = dict(anni=c[0], comm=c[1], eras=c[2], dref=c[3], oper=c[4])
+            rw["rwts"] = sum(c)
+            return buf.value.decode(), rw
 
     def print_book(self, book):
         return self._text(self.lib.oracle_print_book, book)
