@@ -7,6 +7,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -59,6 +60,8 @@ struct hvm_gpu {
   u32 blocks = 0, waves = 0;
   u64 *d_rb_stack = nullptr, *d_rb_out = nullptr;
   u64 *d_batch = nullptr;  // batch staging: roots, def ids, ports
+  u64 *d_captured = nullptr;  // redexes recorded by calls on inert definitions (pairs)
+  u64 captured_cap = 0;       // pairs
   u64 *scratch = nullptr;  // this device's static staging array (hvm_gpu_boot_batch_noalloc)
   // the planned batch (hvm_gpu_batch_plan): device arrays in one buffer
   struct Plan {
@@ -142,6 +145,8 @@ Params make_params(hvm_gpu *h, u64 limit) {
   P.defs = h->d_defs;
   P.instrs = h->d_instrs;
   P.labbits = h->d_labbits;
+  P.captured = h->d_captured;
+  P.captured_cap = h->captured_cap;
   P.trg_buf = h->d_trg;
   P.queue = h->queue;
   P.qmask = h->qslots - 1;
@@ -984,6 +989,7 @@ void hvm_gpu_close(hvm_gpu *h) {
   dfree(h->idle_bits);
   dfree(h->mbox);
   dfree(h->d_batch);
+  dfree(h->d_captured);
   dfree(h->g);
   dfree(h->queue);
   dfree(h->d_defs);
@@ -1016,6 +1022,145 @@ int hvm_gpu_load_book_text(hvm_gpu *h, const char *text) {
     h->labbits_host = h->host.all_labbits();
     upload_book(h);
   });
+}
+
+namespace {
+// The handle's book from `host` (device records, instructions, label bits),
+// uploaded.
+void set_book(hvm_gpu *h, hvm64::Host host) {
+  h->host = std::move(host);
+  h->defs_host = h->host.def_records();
+  h->instrs_host = h->host.all_instrs();
+  h->labbits_host = h->host.all_labbits();
+  upload_book(h);
+}
+}  // namespace
+
+// Book::pre_reduce (transform/src/pre_reduce.rs:38-164).  The order in which
+// definitions are reduced depends on the book alone, so it is found first
+// (pre_reduce.rs:120-163: a definition's references, in tree order, before
+// the definition; a definition met again while its own references are
+// visited keeps its net); then each definition in turn is booted on a fresh
+// net and reduced on the device, the redexes its inert calls captured go
+// back among the pending ones, and the read-back net becomes both the
+// definition's instructions (label sets unchanged) and, in the new book, its
+// net.
+int hvm_gpu_pre_reduce(hvm_gpu *h, const char *book_text, const char *const *names, uint32_t n_names, int only,
+                       uint64_t max_rwts, char *out, uint64_t cap, uint64_t *len, uint64_t rwts[5]) {
+  int rc = 0;
+  int g = guarded(h, [&] {
+    if (!book_text || (n_names && !names)) throw Error(HVM_E_ARG, "null argument");
+    std::set<std::string> listed;
+    for (u32 i = 0; i < n_names; i++) {
+      if (!names[i]) throw Error(HVM_E_ARG, "null name");
+      listed.insert(names[i]);
+    }
+    auto skip = [&](const std::string &x) { return only ? listed.count(x) == 0 : listed.count(x) != 0; };
+    hvm64::Book book;
+    hvm64::Host host;
+    std::vector<std::string> order;
+    hvm64::with_big_stack([&] {
+      book = hvm64::parse_book(book_text);
+      host.insert_book_with_inert_default(book);
+      std::set<std::string> seen;
+      std::function<void(const std::string &)> visit_def;
+      std::function<void(u32)> visit_tree = [&](u32 t) {
+        const hvm64::TNode &n = book.arena.nodes[t];
+        if (n.kind == hvm64::TK::Ref) visit_def(book.arena.names[n.name]);
+        if (n.kind == hvm64::TK::Ctr || n.kind == hvm64::TK::Op || n.kind == hvm64::TK::Switch) {
+          visit_tree(n.a);
+          visit_tree(n.b);
+        }
+      };
+      visit_def = [&](const std::string &name) {
+        if (seen.count(name) || skip(name) || !book.nets.count(name)) return;
+        seen.insert(name);
+        const hvm64::Net &net = book.nets.at(name);
+        visit_tree(net.root);
+        for (auto &r : net.redexes) {
+          visit_tree(r.first);
+          visit_tree(r.second);
+        }
+        order.push_back(name);
+      };
+      for (auto &kv : book.nets) visit_def(kv.first);
+    });
+    if (!h->d_captured) {
+      h->captured_cap = 1 << 20;
+      hip_check(hipMalloc(&h->d_captured, h->captured_cap * 16), "hipMalloc captured redexes");
+    }
+    set_book(h, host);
+    u64 total[5] = {};
+    std::map<std::string, std::string> reduced;
+    bool stale = false;  // the device book lags the host's instructions
+    for (const std::string &name : order) {
+      if (stale) {
+        h->defs_host = h->host.def_records();
+        h->instrs_host = h->host.all_instrs();
+        upload_book(h);
+        stale = false;
+      }
+      reset_net(h, false);
+      Params P = make_params(h, ~0ull);
+      hip_check(launch_exec(P, h->host.def_id(name), mkport(T_VAR, 0, h->root_cell), 1, h->stream), "launch boot");
+      sync_globals(h);
+      // reduce(limit) interacts once before it compares (net.rs:41-52)
+      run_reduce(h, std::max<u64>(max_rwts, 1));
+      for (int k = 0; k < 5; k++) total[k] += h->last.rw[k];
+      // the captured redexes after the pending ones (pre_reduce.rs:150-153)
+      Globals gl;
+      hip_check(hipMemcpy(&gl, h->g, sizeof(Globals), hipMemcpyDeviceToHost), "read globals");
+      if (gl.captured_n > h->captured_cap) throw Error(HVM_E_OOM, "pre-reduce: too many captured redexes");
+      const u64 tail = std::max(gl.q_head, gl.q_tail);
+      if (tail - gl.q_head + gl.captured_n > h->qslots) throw Error(HVM_E_OOM, "pre-reduce: redex ring full");
+      for (u64 i = 0; i < gl.captured_n; i++)
+        hip_check(hipMemcpy(h->queue + ((tail + i) % h->qslots) * 2, h->d_captured + 2 * i, 16,
+                            hipMemcpyDeviceToDevice),
+                  "captured redexes");
+      if (gl.captured_n) {
+        const u64 nt = tail + gl.captured_n;
+        hip_check(hipMemcpy(&h->g->q_tail, &nt, 8, hipMemcpyHostToDevice), "captured redexes");
+      }
+      std::string text = readback(h, h->root_cell, true);
+      hvm64::with_big_stack([&] {
+        hvm64::Arena arena;
+        hvm64::Net net = hvm64::parse_net(arena, text);
+        h->host.set_code(h->host.def_id(name), h->host.encode(arena, net));
+      });
+      reduced[name] = text;
+      stale = true;
+    }
+    std::string result;
+    hvm64::with_big_stack([&] {
+      for (auto &kv : book.nets) {  // ast.rs:221-232
+        if (!result.empty()) result += "\n\n";
+        auto it = reduced.find(kv.first);
+        result += "@" + kv.first + " = " + (it != reduced.end() ? it->second : hvm64::show_net(book.arena, kv.second));
+      }
+      // the handle now holds the new book, its label sets recomputed (the
+      // reference's Host::new on the pre-reduced book); names it still
+      // refers to but does not define stay inert
+      hvm64::Book nb = hvm64::parse_book(result);
+      hvm64::Host nh;
+      nh.insert_book_with_inert_default(nb);
+      host = std::move(nh);
+    });
+    set_book(h, std::move(host));
+    reset_net(h, false);
+    if (rwts)
+      for (int k = 0; k < 5; k++) rwts[k] = total[k];
+    if (len) *len = result.size();
+    if (!out || cap < result.size() + 1) {
+      rc = HVM_E_TRUNC;
+      if (out && cap) {
+        std::memcpy(out, result.data(), cap - 1);
+        out[cap - 1] = 0;
+      }
+      return;
+    }
+    std::memcpy(out, result.data(), result.size() + 1);
+  });
+  return g ? g : rc;
 }
 
 int hvm_gpu_load_book(hvm_gpu *h, const hvm_book_blob *book) {

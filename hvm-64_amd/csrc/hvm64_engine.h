@@ -61,8 +61,12 @@ struct DefRec {
   u32 labs_words; // bitset length in words
   u32 min_safe;   // LabSet::min_safe (also the label of Ref ports to this def)
   u32 group;      // call ring of this definition's calls (by instruction count)
-  u32 pad;
+  u32 flags;      // DEF_INERT: a name the book refers to but does not define
 };
+// An inert definition (transform/src/pre_reduce.rs:99-116) has no
+// instructions; a call on it counts a dereference and records the redex in
+// the captured list instead of expanding anything.
+static const u32 DEF_INERT = 1;
 
 // Rewrite counter slots (runtime.rs:72-79) and interpreter shortcut counters
 // (ERA targets of do_op / do_op_num / do_switch, instruction.rs:144-197),
@@ -116,7 +120,8 @@ struct alignas(128) Globals {
   u32 err;         // first error code
   u32 err_info;
   u64 result;      // kernel-specific result word (new root cell, readback length)
-  u64 _p5[14];
+  u64 captured_n;  // redexes recorded by calls on inert definitions
+  u64 _p5[13];
   u64 rw[RW_SLOTS];  // cumulative counters (RW_POPS: redexes popped)
   u64 _p6[16 - RW_SLOTS];
   u64 st[ST_SLOTS];  // scheduler statistics of the last reduce launch
@@ -134,6 +139,8 @@ struct Params {
   const DefRec *defs;
   const u64 *instrs;   // 2 words per instruction
   const u64 *labbits;
+  u64 *captured;       // pairs recorded by calls on inert definitions
+  u64 captured_cap;    // pairs the list holds
   u64 *trg_buf;        // per-wave trg registers: slot-major, 64 lanes interleaved
   u64 *queue;          // global redex ring, 2 words per slot
   u64 qmask;           // ring slots - 1

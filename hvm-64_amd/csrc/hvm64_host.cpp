@@ -434,6 +434,12 @@ void LabSet::unite(const LabSet &o) {
   if (o.bits.size() > bits.size()) bits.resize(o.bits.size(), 0);
   for (size_t i = 0; i < o.bits.size(); i++) bits[i] |= o.bits[i];
 }
+LabSet LabSet::all() {
+  LabSet s;
+  s.min_safe = 0xFFFF;
+  s.bits.assign(1024, ~0ull);
+  return s;
+}
 bool LabSet::has(u16 lab) const {
   if (lab >= min_safe) return false;
   size_t w = lab >> 6;
@@ -443,6 +449,7 @@ bool LabSet::has(u16 lab) const {
 namespace {
 struct LabelPass {
   const Book &book;
+  const std::function<LabSet(const std::string &)> &missing;
   static constexpr size_t SECOND = SIZE_MAX;  // depth "None": the second phase
   struct Memo {
     bool done;
@@ -491,8 +498,10 @@ struct LabelPass {
     if (n.kind == TK::Ctr && out) out->add(n.lab);
     if (n.kind == TK::Ref) {
       const std::string &nm = book.arena.names[n.name];
-      if (!book.nets.count(nm)) throw Error(HVM_E_NOREF, "Found reference @" + nm + ", which is not in the book!");
-      return def(nm, depth == SECOND ? SECOND : depth + 1, out);
+      if (book.nets.count(nm)) return def(nm, depth == SECOND ? SECOND : depth + 1, out);
+      if (!missing) throw Error(HVM_E_NOREF, "Found reference @" + nm + ", which is not in the book!");
+      if (out) out->unite(missing(nm));
+      return SIZE_MAX;
     }
     if (n.kind == TK::Ctr || n.kind == TK::Op || n.kind == TK::Switch)
       return std::min(tree(n.a, depth, out), tree(n.b, depth, out));
@@ -501,8 +510,9 @@ struct LabelPass {
 };
 }  // namespace
 
-std::map<std::string, LabSet> calculate_label_sets(const Book &book) {
-  LabelPass pass{book, {}};
+std::map<std::string, LabSet> calculate_label_sets(const Book &book,
+                                                   const std::function<LabSet(const std::string &)> &missing) {
+  LabelPass pass{book, missing, {}};
   for (auto &kv : book.nets) pass.def(kv.first, 0, nullptr);
   std::map<std::string, LabSet> out;
   for (auto &kv : pass.memo) out[kv.first] = kv.second.set;
@@ -678,7 +688,27 @@ DefCode Host::encode(const Arena &A, const Net &n) const {
 }
 
 void Host::insert_book(const Book &book) {  // host.rs:41-73
-  auto sets = calculate_label_sets(book);
+  insert_sets(book, calculate_label_sets(book));
+}
+
+void Host::insert_book_with_inert_default(const Book &book) {
+  // the default definition is inserted when the label pass first meets the
+  // name, as host.rs:53-59 does
+  auto sets = calculate_label_sets(book, [&](const std::string &name) {
+    auto it = index_.find(name);
+    if (it != index_.end()) return labs_[it->second];
+    const u32 id = (u32)names_.size();
+    names_.push_back(name);
+    index_[name] = id;
+    labs_.push_back(LabSet::all());
+    code_.emplace_back();
+    inert_.push_back(true);
+    return labs_[id];
+  });
+  insert_sets(book, sets);
+}
+
+void Host::insert_sets(const Book &book, const std::map<std::string, LabSet> &sets) {
   for (auto &kv : sets) {
     if (index_.count(kv.first)) throw Error(HVM_E_ARG, "definition @" + kv.first + " already loaded");
     u32 id = (u32)names_.size();
@@ -686,6 +716,7 @@ void Host::insert_book(const Book &book) {  // host.rs:41-73
     index_[kv.first] = id;
     labs_.push_back(kv.second);
     code_.emplace_back();
+    inert_.push_back(false);
   }
   for (auto &kv : book.nets) code_[index_.at(kv.first)] = encode(book.arena, kv.second);
 }
@@ -713,6 +744,7 @@ std::vector<DefRec> Host::def_records() const {
     r.labs_off = loff;
     r.labs_words = (u32)labs_[i].bits.size();
     r.min_safe = labs_[i].min_safe;
+    r.flags = inert_[i] ? hvm::DEF_INERT : 0;
     ioff += r.ilen;
     loff += r.labs_words;
     recs.push_back(r);
@@ -736,6 +768,7 @@ void Host::adopt(std::vector<std::string> names, std::vector<LabSet> labs, std::
   names_ = std::move(names);
   labs_ = std::move(labs);
   code_ = std::move(code);
+  inert_.assign(names_.size(), false);
   index_.clear();
   for (u32 i = 0; i < names_.size(); i++) {
     if (index_.count(names_[i])) throw Error(HVM_E_ARG, "duplicate definition @" + names_[i]);

@@ -85,6 +85,7 @@ struct LabSet {
   void add(u16 lab);
   void unite(const LabSet &o);
   bool has(u16 lab) const;
+  static LabSet all();  // LabSet::ALL (def.rs:17): every label below 0xFFFF
 };
 
 struct DefCode {
@@ -96,7 +97,15 @@ struct DefCode {
 class Host {
  public:
   // insert_book_with_default (host.rs:41-73): label sets first, then code.
+  // insert_book: a reference to a name the book does not define is an
+  // error; insert_book_with_inert_default: it becomes an inert definition
+  // (LabSet::ALL, no instructions; transform/src/pre_reduce.rs:44-45).
   void insert_book(const Book &book);
+  void insert_book_with_inert_default(const Book &book);
+  bool inert(u32 id) const { return inert_[id]; }
+  // Replaces a definition's instructions; its label set stays
+  // (pre_reduce.rs:156-157).
+  void set_code(u32 id, DefCode code) { code_[id] = std::move(code); }
   bool has_def(const std::string &name) const { return index_.count(name) != 0; }
   u32 def_id(const std::string &name) const;
   const std::string &def_name(u32 id) const { return names_[id]; }
@@ -121,6 +130,8 @@ class Host {
   std::map<std::string, u32> index_;
   std::vector<LabSet> labs_;
   std::vector<DefCode> code_;
+  std::vector<bool> inert_;
+  void insert_sets(const Book &book, const std::map<std::string, LabSet> &sets);
 };
 
 // A definition compiled into a residual program and a template
@@ -145,8 +156,11 @@ struct DeviceBook {
 };
 DeviceBook build_device_book(std::vector<hvm::DefRec> &defs, const std::vector<u64> &instrs);
 
-// Label sets of every definition (calc_labels.rs:78-194).
-std::map<std::string, LabSet> calculate_label_sets(const Book &book);
+// Label sets of every definition (calc_labels.rs:78-194).  A reference to a
+// name the book does not define takes its set from `missing`
+// (calc_labels.rs:183-190), or is an error when `missing` is empty.
+std::map<std::string, LabSet> calculate_label_sets(
+    const Book &book, const std::function<LabSet(const std::string &)> &missing = {});
 
 // Rebuild normal-form text from the device's preorder serialisation
 // (k_readback): principal ports in preorder, Var(cell) for aux-aux wires.

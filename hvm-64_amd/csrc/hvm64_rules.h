@@ -780,7 +780,7 @@ DEV DefRec load_def(const Lane &L, u32 di) {
     d.labs_words = (u32)w2;
     d.min_safe = (u32)(w2 >> 32);
     d.group = (u32)L.book[4 * di + 3];
-    d.pad = 0;
+    d.flags = (u32)(L.book[4 * di + 3] >> 32);
     return d;
   }
   return L.P->defs[di];
@@ -1159,6 +1159,19 @@ DEV void expand_call(Lane &L, Interp &I, u32 &np, u64 cell, u64 port) {
 }
 
 // def.rs:177-189
+// Records (port, trg) in the captured list; the host puts the list back
+// among the net's pending redexes after the launch.
+DEV void capture_redex(Lane &L, u64 port, u64 trg) {
+  const Params &P = *L.P;
+  const u64 i = HVM_FAA(&P.g->captured_n, 1ull);
+  if (i >= P.captured_cap) {
+    fail(L, ERR_OOM);
+    return;
+  }
+  HVM_ST(&P.captured[2 * i], port);
+  HVM_ST(&P.captured[2 * i + 1], trg);
+}
+
 DEV void call(Lane &L, Interp &I, u32 &np, u64 port, u64 trg) {
   const Params &P = *L.P;
   u64 addr = paddr(port);
@@ -1182,6 +1195,10 @@ DEV void call(Lane &L, Interp &I, u32 &np, u64 port, u64 trg) {
     return;
   }
   L.rw[RW_DREF]++;
+  if (d.flags & DEF_INERT) {  // InertDef::call (pre_reduce.rs:108-115)
+    capture_redex(L, port, trg);
+    return;
+  }
   interp_begin(L, I, d, trg, load_cdef(L, di));
 }
 

@@ -36,7 +36,7 @@ EXPORTS = [
     "hvm_gpu_boot_batch_noalloc", "hvm_gpu_batch_plan", "hvm_gpu_batch_run", "hvm_gpu_batch_tune",
     "hvm_gpu_batch_info", "hvm_gpu_batch_used", "hvm_gpu_alloc_count",
     "hvm_gpu_serve_start", "hvm_gpu_serve_submit", "hvm_gpu_serve_poll", "hvm_gpu_serve_stop",
-    "hvm_gpu_snapshot", "hvm_gpu_load_book_image", "hvm_gpu_set_compiled",
+    "hvm_gpu_snapshot", "hvm_gpu_load_book_image", "hvm_gpu_set_compiled", "hvm_gpu_pre_reduce",
     "hvm_book_print", "hvm_book_label_sets", "hvm_net_apply_tree", "hvm_book_encode",
 ]
 
@@ -123,6 +123,9 @@ def load_library(path=None):
     lib.hvm_gpu_batch_info.argtypes = [P, ctypes.POINTER(BatchTuning), ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(u64)]
     lib.hvm_gpu_alloc_count.argtypes = [P, ctypes.POINTER(u64)]
+    lib.hvm_gpu_pre_reduce.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), ctypes.c_uint32,
+                                       ctypes.c_int, u64, ctypes.c_char_p, u64, ctypes.POINTER(u64),
+                                       ctypes.POINTER(u64)]
     lib.hvm_gpu_batch_used.argtypes = [P, ctypes.POINTER(u64)]
     lib.hvm_gpu_serve_start.argtypes = [P, ctypes.c_uint32, u64, u64]
     lib.hvm_gpu_serve_submit.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(u64)]
@@ -553,6 +556,28 @@ class Serving:
         self.running = False
         self.heap._check(self.heap._lib.hvm_gpu_serve_stop(self.heap._h, ctypes.byref(ms)))
         return ms.value
+
+
+def pre_reduce(heap, book, skip=(), only=None, max_rwts=(1 << 64) - 1):
+    """Book::pre_reduce (transform/src/pre_reduce.rs:38-75) with every
+    definition's reduction on the device: reduces the definitions of `book`
+    (text) except those in `skip` (or, with `only`, just those), each for at
+    most `max_rwts` interactions.  Returns (new book text, Rewrites of the
+    pre-reduction); the heap then holds the new book."""
+    names = list(only if only is not None else skip)
+    arr = (ctypes.c_char_p * max(len(names), 1))(*[x.encode() for x in names])
+    cap = 4 * len(book) + (1 << 20)
+    while True:
+        buf = ctypes.create_string_buffer(cap)
+        n = ctypes.c_uint64()
+        c = (ctypes.c_uint64 * 5)()
+        rc = heap._lib.hvm_gpu_pre_reduce(heap._h, book.encode(), arr, len(names), int(only is not None), max_rwts,
+                                          buf, cap, ctypes.byref(n), c)
+        if rc == 9 and n.value + 1 > cap:
+            cap = n.value + 1
+            continue
+        heap._check(rc)
+        return buf.value.decode(), Rewrites(*[int(x) for x in c])
 
 
 def alloc_count(heap):

@@ -235,6 +235,22 @@ int hvm_gpu_serve_submit(hvm_gpu *h, uint32_t n, const uint32_t *def_ids, const 
 int hvm_gpu_serve_poll(hvm_gpu *h, uint64_t *tags, uint64_t *ports, uint32_t cap, uint32_t *n);
 int hvm_gpu_serve_stop(hvm_gpu *h, double *ms);
 
+/* ~ Book::pre_reduce (transform/src/pre_reduce.rs:38-164): reduces each
+ * definition of `book` on its own net on the device, for at most max_rwts
+ * interactions (reduce(limit), net.rs:41-52), its references first, and
+ * writes the book with every reduced definition's read-back net (pending
+ * redexes included) in its place.  `names` (n_names of them) are the
+ * definitions to leave as they are, or, with only != 0, the only ones to
+ * reduce (the reference's `skip` predicate).  A name the book refers to but
+ * does not define is inert: a call on it counts a dereference and the redex
+ * stays pending (pre_reduce.rs:44-45, 99-116); it is not written out.
+ * rwts (may be null) receives the pre-reduction's counts (anni comm eras
+ * dref oper).  The handle then holds the new book, label sets recomputed
+ * (Host::new on it; names it refers to but does not define stay inert);
+ * buffer protocol as hvm_gpu_readback. */
+int hvm_gpu_pre_reduce(hvm_gpu *h, const char *book, const char *const *names, uint32_t n_names, int only,
+                       uint64_t max_rwts, char *out, uint64_t cap, uint64_t *len, uint64_t rwts[5]);
+
 /* ~ Net.rwts (runtime/src/runtime.rs:72-79): anni, comm, eras, dref, oper. */
 int hvm_gpu_rewrites(hvm_gpu *h, uint64_t out[5]);
 
