@@ -826,60 +826,156 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
 // pending redex in global-ring slots [q_head, q_tail) (after a reduce(limit),
 // every pending redex is there).  Principal ports are emitted as they are; a
 // wire between two aux ports is emitted as Var(cell) of the var cell that
-// joins them.  g->result = number of entries, or ERR_READBACK.
-__global__ void k_readback(Params P, u64 root_cell, u64 q_head, u64 q_tail, u64 *stack, u64 stack_cap, u64 *out,
-                           u64 out_cap) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  u64 n = 0;
-  // the root's tree, then both sides of every pending redex in ring order
-  // (Host::readback reads rt_net.redexes after the root, readback.rs:28-30)
-  for (u64 k = 0; k < 1 + 2 * (q_tail - q_head); k++) {
-    u64 sp = 0;
+// joins them.  g->result = number of entries, or ERR_READBACK (err_info 1:
+// `work` or `out` too small; 2: a dangling or cyclic wire).
+//
+// One block, level by level, so a wide tree is read with its lanes in
+// parallel:
+//  1. breadth first: the lanes of a level resolve its entries (following
+//     parked ports) and append each binary node's two children, aux port 1
+//     first, as a pair;
+//  2. bottom up: the size of each entry's subtree;
+//  3. top down: each entry's preorder position (the trees one after another;
+//     a node's first child right after it, its second child after the first
+//     child's subtree), where its port is written.
+// `work` holds four arrays of M = (work_cap - 1) / 4 words: ports, first
+// child (NO_KID: none), sizes (then positions), and the levels' first
+// entries.
+static const u32 RB_THREADS = 256;
+static const u64 NO_KID = ~0ull;
+
+__global__ __launch_bounds__(RB_THREADS) void k_readback(Params P, u64 root_cell, u64 q_head, u64 q_tail, u64 *work,
+                                                         u64 work_cap, u64 *out, u64 out_cap) {
+  const u32 t = threadIdx.x;
+  const u64 M = (work_cap - 1) / 4;
+  u64 *const port = work, *const kid = work + M, *const sz = work + 2 * M, *const lev = work + 3 * M;
+  __shared__ unsigned long long s_next;  // entries appended so far
+  __shared__ u32 s_err;
+  __shared__ u64 s_n;
+  const u64 nroots = 1 + 2 * (q_tail - q_head);
+  if (t == 0) {
+    s_next = nroots;
+    s_err = nroots > M ? 1 : 0;
+  }
+  __syncthreads();
+  if (s_err) {
+    if (t == 0) {
+      P.g->err_info = 1;
+      P.g->err = ERR_READBACK;
+      P.g->result = 0;
+    }
+    return;
+  }
+  for (u64 k = t; k < nroots; k += RB_THREADS) {
+    u64 p;
     if (k == 0) {
-      stack[sp++] = var_port(root_cell);
+      p = var_port(root_cell);
     } else {
       const u64 *slot = P.queue + ((q_head + (k - 1) / 2) & P.qmask) * 2;
-      stack[sp++] = gld((u64 *)slot + ((k - 1) & 1));
+      p = gld((u64 *)slot + ((k - 1) & 1));
     }
-    while (sp > 0) {
-      u64 p = stack[--sp];
+    HVM_ST(&port[k], p);
+  }
+  __syncthreads();
+  // 1. breadth first
+  u64 a = 0, b = nroots, L = 0;
+  while (a < b) {
+    if (t == 0) HVM_ST(&lev[L], a);
+    for (u64 i = a + t; i < b; i += RB_THREADS) {
+      u64 p = HVM_LD(&port[i]);
       u64 hops = 0;
+      bool bad = false;
       while (is_endpoint(p)) {
-        u64 v = ld(P, paddr(p));
+        const u64 v = ld(P, paddr(p));
         if (v == P_NONE) break;
         if (v == P_FREE || ++hops > (1ull << 24)) {
-          P.g->err_info = 2;  // dangling or cyclic wire
-          P.g->err = ERR_READBACK;
-          P.g->result = n;
-          return;
+          bad = true;
+          break;
         }
         p = v;
       }
-      if (n >= out_cap) {
-        P.g->err_info = 1;  // buffer too small
-        P.g->err = ERR_READBACK;
-        P.g->result = n;
-        return;
-      }
-      if (is_endpoint(p)) {
-        out[n++] = var_port(paddr(p));
-        continue;
-      }
-      out[n++] = p;
-      if (is_binary(p)) {
-        if (sp + 2 > stack_cap) {
-          P.g->err_info = 1;
-          P.g->err = ERR_READBACK;
-          P.g->result = n;
-          return;
+      u64 k = NO_KID;
+      if (bad) {
+        atomicMax(&s_err, 2u);
+      } else if (is_endpoint(p)) {
+        p = var_port(paddr(p));
+      } else if (is_binary(p)) {
+        k = atomicAdd(&s_next, 2ull);
+        if (k + 2 > M) {
+          atomicMax(&s_err, 1u);
+          k = NO_KID;
+        } else {
+          const u64 ad = paddr(p);
+          HVM_ST(&port[k], ld(P, ad));
+          HVM_ST(&port[k + 1], ld(P, ad + 8));
         }
-        u64 a = paddr(p);
-        stack[sp++] = ld(P, a + 8);
-        stack[sp++] = ld(P, a);
+      }
+      HVM_ST(&port[i], p);
+      HVM_ST(&kid[i], k);
+    }
+    __syncthreads();
+    if (s_err) break;
+    a = b;
+    b = s_next;
+    L++;
+    __syncthreads();
+  }
+  if (s_err) {
+    if (t == 0) {
+      P.g->err_info = s_err;
+      P.g->err = ERR_READBACK;
+      P.g->result = 0;
+    }
+    return;
+  }
+  if (t == 0) HVM_ST(&lev[L], a);  // a: every entry
+  __syncthreads();
+  // 2. bottom up: subtree sizes
+  for (u64 l = L; l-- > 0;) {
+    const u64 s0 = HVM_LD(&lev[l]), s1 = HVM_LD(&lev[l + 1]);
+    for (u64 i = s0 + t; i < s1; i += RB_THREADS) {
+      const u64 k = HVM_LD(&kid[i]);
+      u64 z = 1;
+      if (k != NO_KID) z += HVM_LD(&sz[k]) + HVM_LD(&sz[k + 1]);
+      HVM_ST(&sz[i], z);
+    }
+    __syncthreads();
+  }
+  // 3. top down: positions (the trees in order), each port to its place
+  if (t == 0) {
+    u64 off = 0;
+    for (u64 r = 0; r < nroots; r++) {
+      const u64 z = HVM_LD(&sz[r]);
+      HVM_ST(&sz[r], off);
+      off += z;
+    }
+    s_n = off;
+    s_err = off > out_cap ? 1 : 0;
+  }
+  __syncthreads();
+  if (s_err) {
+    if (t == 0) {
+      P.g->err_info = 1;
+      P.g->err = ERR_READBACK;
+      P.g->result = s_n;
+    }
+    return;
+  }
+  for (u64 l = 0; l < L; l++) {
+    const u64 s0 = HVM_LD(&lev[l]), s1 = HVM_LD(&lev[l + 1]);
+    for (u64 i = s0 + t; i < s1; i += RB_THREADS) {
+      const u64 pos = HVM_LD(&sz[i]);
+      HVM_ST(&out[pos], HVM_LD(&port[i]));
+      const u64 k = HVM_LD(&kid[i]);
+      if (k != NO_KID) {
+        const u64 zk = HVM_LD(&sz[k]);  // still the first child's size
+        HVM_ST(&sz[k], pos + 1);
+        HVM_ST(&sz[k + 1], pos + 1 + zk);
       }
     }
+    __syncthreads();
   }
-  P.g->result = n;
+  if (t == 0) P.g->result = s_n;
 }
 
 // ---------------------------------------------------------------------------
@@ -966,8 +1062,8 @@ hipError_t launch_reduce(const Params &P, u32 blocks, hipStream_t st) {
 }
 hipError_t launch_readback(const Params &P, u64 root_cell, u64 q_head, u64 q_tail, u64 *stack, u64 stack_cap,
                            u64 *out, u64 out_cap, hipStream_t st) {
-  hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, st, P, root_cell, q_head, q_tail, stack, stack_cap, out,
-                     out_cap);
+  hipLaunchKernelGGL(k_readback, dim3(1), dim3(RB_THREADS), 0, st, P, root_cell, q_head, q_tail, stack, stack_cap,
+                     out, out_cap);
   return hipGetLastError();
 }
 u32 reduce_waves_per_block() { return WAVES_PER_BLOCK; }

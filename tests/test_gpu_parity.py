@@ -580,3 +580,21 @@ def test_era_shortcuts_interpreted(gpu_heap, oracle, text, shortcuts):
         assert net.shortcuts == ref["shortcuts"] == shortcuts
     finally:
         gpu_heap.set_compiled(True)
+
+
+def test_large_normal_form_readback(gpu_heap, oracle):
+    """A normal form that is a wide tree (the 2^12 keys the bitonic sort
+    generates, about 28k nodes) read back level by level on the device is the
+    oracle's text (readback.rs:23-87 order, variable names by first
+    occurrence), with the same counts."""
+    from hvm64 import configs
+    text = configs.bitonic(12, 7)
+    main = "@main = a\n& @sum ~ (b a)\n& @sort ~ (c (0 b))\n& @rev ~ (d c)\n& @gen ~ (12 (0 d))"
+    assert main in text
+    text = text.replace(main, "@main = a\n& @gen ~ (12 (0 a))")
+    print("case large readback: the generated tree of 2^12 keys", flush=True)
+    r = oracle.run(text, heap_words=1 << 26)
+    assert r["nf"].count("(") > 4000
+    nf, rw, _ = run_gpu(gpu_heap, text)
+    assert nf == r["nf"]
+    assert rw == r["rwts"]
