@@ -105,6 +105,7 @@ struct hvm_gpu {
   } serve;
   bool heap_dirty = false;  // arenas left words the allocation rings would read as live
   u64 timeout_ticks = 0;  // wall-clock ticks a reduction may run (0 = unbounded)
+  bool sched_timing = false;  // scheduler statistics with wall-clock ticks
   u64 wall_khz = 100000;
   std::string err;
 };
@@ -145,6 +146,7 @@ Params make_params(hvm_gpu *h, u64 limit) {
   P.defs = h->d_defs;
   P.instrs = h->d_instrs;
   P.labbits = h->d_labbits;
+  P.timing = h->sched_timing ? 1 : 0;
   P.captured = h->d_captured;
   P.captured_cap = h->captured_cap;
   P.trg_buf = h->d_trg;
@@ -1598,6 +1600,13 @@ int hvm_gpu_sched_stats(hvm_gpu *h, uint64_t out[20]) {
     out[10] = h->last.rw[RW_OV_ALLOC];
     for (u32 k = 0; k < 8; k++) out[11 + k] = h->last.st[ST_CLS_IT + k];
     out[19] = h->last.rw[RW_POPS];
+  });
+}
+
+int hvm_gpu_set_sched_timing(hvm_gpu *h, int on) {
+  return guarded(h, [&] {
+    h->sched_timing = on != 0;
+    replan(h, h->plan.tuning);  // a plan's graph captured the launch's parameters
   });
 }
 

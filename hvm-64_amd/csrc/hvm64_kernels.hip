@@ -335,7 +335,10 @@ DEV void clamp_rings(u32 *top_c, const u32 *bot_c, u32 lane) {
 // allocation rings; each instantiation carries only its mode's allocator.
 // SERVE: the serving launch (arenas are request slots; the kernel runs until
 // the host sets the stop flag and no work is left).
-template <bool ARENAS, bool SERVE>
+// TIMING: the scheduler statistics include wall-clock ticks (the batched
+// and serving instantiations always; the single-net one when asked for, as
+// the clock reads and the tick sums cost it about 2%).
+template <bool ARENAS, bool SERVE, bool TIMING>
 __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   __shared__ u64 s_bag[WAVES_PER_BLOCK][RING_PAIRS * 2];
   __shared__ u64 s_T[WAVES_PER_BLOCK][(LDS_SLOTS + 2 * LINK_CAP) * 64];
@@ -650,7 +653,8 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     }
 
     // ---- pop (LIFO) and interact
-    const u64 t_it = wall_clock64();
+    const bool timing = TIMING;
+    const u64 t_it = timing ? wall_clock64() : 0;
     st[ST_BUSY_IT]++;
     const u32 nth = (lane - rot) & 63;  // this lane runs the nth-newest redex
     bool has = nth < take;
@@ -664,11 +668,11 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) top_c[c] = top - take;
     __builtin_amdgcn_wave_barrier();
-    const u64 t_in = wall_clock64();
+    const u64 t_in = timing ? wall_clock64() : 0;
     if (has) L.rw[RW_POPS]++;
     run_task(L, has, ra, rb, NO_DEF, 0);
     {
-      const u64 dt = wall_clock64() - t_in;
+      const u64 dt = timing ? wall_clock64() - t_in : 0;
       st[ST_INTERACT_TICKS] += dt;
       // c is wave-uniform; unrolled so st stays in registers
       const u32 sk = ring_stat(c);
@@ -767,13 +771,13 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
       }
     }
     if (spill) {
-      const u64 t_sp = wall_clock64();
+      const u64 t_sp = timing ? wall_clock64() : 0;
       st[ST_SPILLS]++;
       st[ST_SPILLED] += spill;
       u32 qerr = 0;
       S_cache = spill_to_global(P, bag, &bot_c[sc], sc, spill, lane, uniform(__shfl(my_bot, sc, 64)), qerr);
       if (qerr) fail(L, qerr);
-      st[ST_SPILL_TICKS] += wall_clock64() - t_sp;
+      if (timing) st[ST_SPILL_TICKS] += wall_clock64() - t_sp;
     }
     // ---- batched nets: a busy wave short of work also claims from the
     // global ring, while few waves are idle.  With n < 64 pending, every
@@ -808,7 +812,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
       }
     }
     __builtin_amdgcn_wave_barrier();
-    st[ST_BUSY_TICKS] += wall_clock64() - t_it;
+    if (timing) st[ST_BUSY_TICKS] += wall_clock64() - t_it;
   }
 
   lane_finish(L);
@@ -1055,9 +1059,11 @@ hipError_t launch_root_ports(const Params &P, u32 n, const u64 *roots, u64 *out,
   return hipGetLastError();
 }
 hipError_t launch_reduce(const Params &P, u32 blocks, hipStream_t st) {
-  if (P.serve) hipLaunchKernelGGL((k_reduce<true, true>), dim3(blocks), dim3(WAVES_PER_BLOCK * 64), 0, st, P);
-  else if (P.gmap) hipLaunchKernelGGL((k_reduce<true, false>), dim3(blocks), dim3(WAVES_PER_BLOCK * 64), 0, st, P);
-  else hipLaunchKernelGGL((k_reduce<false, false>), dim3(blocks), dim3(WAVES_PER_BLOCK * 64), 0, st, P);
+  const dim3 grid(blocks), block(WAVES_PER_BLOCK * 64);
+  if (P.serve) hipLaunchKernelGGL((k_reduce<true, true, true>), grid, block, 0, st, P);
+  else if (P.gmap) hipLaunchKernelGGL((k_reduce<true, false, true>), grid, block, 0, st, P);
+  else if (P.timing) hipLaunchKernelGGL((k_reduce<false, false, true>), grid, block, 0, st, P);
+  else hipLaunchKernelGGL((k_reduce<false, false, false>), grid, block, 0, st, P);
   return hipGetLastError();
 }
 hipError_t launch_readback(const Params &P, u64 root_cell, u64 q_head, u64 q_tail, u64 *stack, u64 stack_cap,
@@ -1070,7 +1076,7 @@ u32 reduce_waves_per_block() { return WAVES_PER_BLOCK; }
 u32 reduce_book_lds_words() { return BOOK_LDS_WORDS; }
 u32 reduce_lds_slots() { return LDS_SLOTS; }
 hipError_t reduce_occupancy(int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_reduce<false, false>, WAVES_PER_BLOCK * 64, 0);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_reduce<false, false, false>, WAVES_PER_BLOCK * 64, 0);
 }
 
 }  // namespace hvm

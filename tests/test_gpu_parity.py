@@ -598,3 +598,29 @@ def test_large_normal_form_readback(gpu_heap, oracle):
     nf, rw, _ = run_gpu(gpu_heap, text)
     assert nf == r["nf"]
     assert rw == r["rwts"]
+
+
+def test_sched_timing_switch(gpu_heap, oracle):
+    """Scheduler tick statistics only when asked for (hvm_gpu_set_sched_timing);
+    iteration and pop counts either way, and the same result."""
+    import hvm64
+    from hvm64 import configs
+    text = configs.tree_sum(12)
+    print("case sched timing on/off: tree-sum 12", flush=True)
+    r = oracle.run(text, heap_words=1 << 26)
+    host = hvm64.Host(text, gpu_heap)
+    out = {}
+    try:
+        for on in (False, True):
+            gpu_heap.set_sched_timing(on)
+            net = hvm64.Net(gpu_heap)
+            net.boot(host.defs["main"])
+            net.normal()
+            assert host.readback(net) == r["nf"]
+            assert net.rwts.as_dict() == r["rwts"]
+            out[on] = net.sched_stats()
+    finally:
+        gpu_heap.set_sched_timing(False)
+    assert out[False]["busy_iters"] > 0 and out[False]["pops"] > 0
+    assert out[False]["busy_ticks"] == 0 and out[False]["call_ticks"] == 0
+    assert out[True]["busy_ticks"] > 0 and out[True]["interact_ticks"] > 0
