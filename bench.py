@@ -362,7 +362,7 @@ def bench_serve(args):
     from hvm64 import serve
     heap_bytes = int(args.heap_gb * (1 << 30))
     srv = serve.Server(slots=args.slots, heap_bytes=heap_bytes, book_seed=args.seed)
-    srv.heap.set_sched_timing(args.sched_timing)
+    srv.heap.set_sched_stats(args.sched_stats)
     try:
         if args.warmup:
             srv.run(serve.poisson_requests(args.rate, 0.2, seed=args.seed + 1))
@@ -423,8 +423,8 @@ def main():
     ap.add_argument("--heap-gb", type=float, default=None,
                     help="heap size (default 192 GiB of the MI355X's 288 GB, batch 224 GiB)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sched-timing", action="store_true",
-                    help="tick counts in the scheduler statistics (diagnostic; costs about 2%%)")
+    ap.add_argument("--sched-stats", action="store_true",
+                    help="collect the scheduler statistics of a single net's launch (diagnostic; costs 2-4%%)")
     ap.add_argument("--interpreted", action="store_true",
                     help="run every definition interpreted (compiled definitions off)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -453,7 +453,7 @@ def main():
     size, text, expected, desc = workload(args.workload, args.size, args.seed)
     heap = hvm64.Heap(int(args.heap_gb * (1 << 30)))
     heap.set_timeout(300)
-    heap.set_sched_timing(args.sched_timing)
+    heap.set_sched_stats(args.sched_stats)
     t_up = time.perf_counter()
     host = hvm64.Host(text, heap)  # parse, encode and upload the book
     n_compiled = heap.set_compiled(not args.interpreted)

@@ -105,7 +105,7 @@ struct hvm_gpu {
   } serve;
   bool heap_dirty = false;  // arenas left words the allocation rings would read as live
   u64 timeout_ticks = 0;  // wall-clock ticks a reduction may run (0 = unbounded)
-  bool sched_timing = false;  // scheduler statistics with wall-clock ticks
+  bool sched_stats_on = false;  // a single net's launches collect scheduler statistics
   u64 wall_khz = 100000;
   std::string err;
 };
@@ -146,7 +146,7 @@ Params make_params(hvm_gpu *h, u64 limit) {
   P.defs = h->d_defs;
   P.instrs = h->d_instrs;
   P.labbits = h->d_labbits;
-  P.timing = h->sched_timing ? 1 : 0;
+  P.timing = h->sched_stats_on ? 1 : 0;
   P.captured = h->d_captured;
   P.captured_cap = h->captured_cap;
   P.trg_buf = h->d_trg;
@@ -1603,9 +1603,9 @@ int hvm_gpu_sched_stats(hvm_gpu *h, uint64_t out[20]) {
   });
 }
 
-int hvm_gpu_set_sched_timing(hvm_gpu *h, int on) {
+int hvm_gpu_set_sched_stats(hvm_gpu *h, int on) {
   return guarded(h, [&] {
-    h->sched_timing = on != 0;
+    h->sched_stats_on = on != 0;
     replan(h, h->plan.tuning);  // a plan's graph captured the launch's parameters
   });
 }

@@ -89,9 +89,8 @@ enum : u32 { BAD_HEAP_ADDR = 1, BAD_LINK_LIST, BAD_CHAIN, BAD_RESOLVE, BAD_SLOTS
 // Scheduler statistics (summed over waves): iterations with work, idle
 // iterations, wall-clock ticks (100 MHz) spent in iterations with work,
 // spills to the global ring / redexes spilled, claims / redexes claimed.
-// A single net's launch fills the tick slots only when Params::timing is
-// set (reading the clock and summing ticks in every iteration cost it about
-// 2%); batched and serving launches always fill them.
+// A single net's launch collects them only when Params::timing is set
+// (keeping them costs it 2-4%); batched and serving launches always do.
 enum : u32 {
   ST_BUSY_IT = 0, ST_IDLE_IT, ST_BUSY_TICKS, ST_SPILLS, ST_SPILLED, ST_CLAIMS, ST_CLAIMED,
   ST_INTERACT_TICKS,  // ticks inside interact() (the divergent section)
@@ -163,7 +162,7 @@ struct Params {
   u64 *mbox;           // per-wave mailboxes (MBOX_WORDS each)
   u32 idle_words;      // words in idle_bits
   u32 fill;            // busy waves short of work also claim from the global ring
-  u32 timing;          // single net: the scheduler statistics include wall-clock ticks
+  u32 timing;          // single net: collect the scheduler statistics (ST_*)
   u64 limit;           // redex budget for reduce(limit); ~0 = unlimited
   u64 timeout_ticks;   // wall-clock (100 MHz) budget of a launch; 0 = none
   // scheduler thresholds (tunable): redexes a busy wave keeps before handing

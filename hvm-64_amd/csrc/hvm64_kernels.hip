@@ -335,9 +335,9 @@ DEV void clamp_rings(u32 *top_c, const u32 *bot_c, u32 lane) {
 // allocation rings; each instantiation carries only its mode's allocator.
 // SERVE: the serving launch (arenas are request slots; the kernel runs until
 // the host sets the stop flag and no work is left).
-// TIMING: the scheduler statistics include wall-clock ticks (the batched
-// and serving instantiations always; the single-net one when asked for, as
-// the clock reads and the tick sums cost it about 2%).
+// TIMING: the launch collects scheduler statistics (iterations, spills,
+// claims, wall-clock ticks): the batched and serving instantiations always;
+// the single-net one when asked for, as keeping them costs it 2-4%.
 template <bool ARENAS, bool SERVE, bool TIMING>
 __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   __shared__ u64 s_bag[WAVES_PER_BLOCK][RING_PAIRS * 2];
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
           break;
         }
       }
-      st[ST_IDLE_IT]++;
+      if (TIMING) st[ST_IDLE_IT]++;
       if (use_mbox) {
         u64 k = 0;
         if (lane == 0) k = gld(my_box);
@@ -494,8 +494,8 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
             bag[i + 1] = b;
           }
           if (lane == 0) gst(my_box, 0);
-          st[ST_CLAIMS]++;
-          st[ST_CLAIMED] += k;
+          if (TIMING) st[ST_CLAIMS]++;
+          if (TIMING) st[ST_CLAIMED] += k;
           active = true;
           in_set = false;
           backoff = 1 + (gwave & 7);
@@ -578,8 +578,8 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
             if (k > 0) {
               backoff = 1 + (gwave & 7);
               active = true;
-              st[ST_CLAIMS]++;
-              st[ST_CLAIMED] += k;
+              if (TIMING) st[ST_CLAIMS]++;
+              if (TIMING) st[ST_CLAIMED] += k;
               claim_global(P, L, bag, top_c, k, lane, t_end);  // the rings are empty: k <= 64 fit
               __builtin_amdgcn_wave_barrier();
               continue;
@@ -655,7 +655,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     // ---- pop (LIFO) and interact
     const bool timing = TIMING;
     const u64 t_it = timing ? wall_clock64() : 0;
-    st[ST_BUSY_IT]++;
+    if (TIMING) st[ST_BUSY_IT]++;
     const u32 nth = (lane - rot) & 63;  // this lane runs the nth-newest redex
     bool has = nth < take;
     u64 ra = 0, rb = 0;
@@ -673,14 +673,14 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
     run_task(L, has, ra, rb, NO_DEF, 0);
     {
       const u64 dt = timing ? wall_clock64() - t_in : 0;
-      st[ST_INTERACT_TICKS] += dt;
+      if (TIMING) st[ST_INTERACT_TICKS] += dt;
       // c is wave-uniform; unrolled so st stays in registers
       const u32 sk = ring_stat(c);
 #pragma unroll
       for (u32 k = 0; k < 4; k++)
         if (sk == k) {
-          st[ST_CLS_IT + k]++;
-          st[ST_CLS_TICKS + k] += dt;
+          if (TIMING) st[ST_CLS_IT + k]++;
+          if (TIMING) st[ST_CLS_TICKS + k] += dt;
         }
     }
     if (++busy >= REFRESH) {  // in flight while this iteration's stores retire
@@ -764,16 +764,16 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
             gst(box, (u64)give);
             bot_c[sc] = bot0 + give;
           }
-          st[ST_SPILLS]++;
-          st[ST_SPILLED] += give;
+          if (TIMING) st[ST_SPILLS]++;
+          if (TIMING) st[ST_SPILLED] += give;
           spill = 0;
         }
       }
     }
     if (spill) {
       const u64 t_sp = timing ? wall_clock64() : 0;
-      st[ST_SPILLS]++;
-      st[ST_SPILLED] += spill;
+      if (TIMING) st[ST_SPILLS]++;
+      if (TIMING) st[ST_SPILLED] += spill;
       u32 qerr = 0;
       S_cache = spill_to_global(P, bag, &bot_c[sc], sc, spill, lane, uniform(__shfl(my_bot, sc, 64)), qerr);
       if (qerr) fail(L, qerr);
@@ -806,8 +806,8 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
       k = uniform(__shfl(k, 0, 64));
       S_cache = __shfl(s_new, 0, 64);
       if (k > 0) {
-        st[ST_CLAIMS]++;
-        st[ST_CLAIMED] += k;
+        if (TIMING) st[ST_CLAIMS]++;
+        if (TIMING) st[ST_CLAIMED] += k;
         claim_global(P, L, bag, top_c, k, lane, t_end);
       }
     }

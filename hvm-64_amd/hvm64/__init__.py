@@ -37,7 +37,7 @@ EXPORTS = [
     "hvm_gpu_batch_info", "hvm_gpu_batch_used", "hvm_gpu_alloc_count",
     "hvm_gpu_serve_start", "hvm_gpu_serve_submit", "hvm_gpu_serve_poll", "hvm_gpu_serve_stop",
     "hvm_gpu_snapshot", "hvm_gpu_load_book_image", "hvm_gpu_set_compiled", "hvm_gpu_pre_reduce",
-    "hvm_gpu_set_sched_timing",
+    "hvm_gpu_set_sched_stats",
     "hvm_book_print", "hvm_book_label_sets", "hvm_net_apply_tree", "hvm_book_encode",
 ]
 
@@ -124,7 +124,7 @@ def load_library(path=None):
     lib.hvm_gpu_batch_info.argtypes = [P, ctypes.POINTER(BatchTuning), ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(u64)]
     lib.hvm_gpu_alloc_count.argtypes = [P, ctypes.POINTER(u64)]
-    lib.hvm_gpu_set_sched_timing.argtypes = [P, ctypes.c_int]
+    lib.hvm_gpu_set_sched_stats.argtypes = [P, ctypes.c_int]
     lib.hvm_gpu_pre_reduce.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), ctypes.c_uint32,
                                        ctypes.c_int, u64, ctypes.c_char_p, u64, ctypes.POINTER(u64),
                                        ctypes.POINTER(u64)]
@@ -221,9 +221,9 @@ class Heap:
         self._check(self._lib.hvm_gpu_set_compiled(self._h, 1 if on else 0, ctypes.byref(n)))
         return n.value
 
-    def set_sched_timing(self, on=True):
-        """Tick counts in the scheduler statistics (off by default)."""
-        self._check(self._lib.hvm_gpu_set_sched_timing(self._h, 1 if on else 0))
+    def set_sched_stats(self, on=True):
+        """Scheduler statistics for single-net launches (off by default)."""
+        self._check(self._lib.hvm_gpu_set_sched_stats(self._h, 1 if on else 0))
 
     def grid(self):
         b, w = ctypes.c_uint32(), ctypes.c_uint32()

@@ -307,14 +307,14 @@ int hvm_gpu_grid(hvm_gpu *h, uint32_t *blocks, uint32_t *waves);
  * slots found in use on a later lap, [10] overflow-ring allocations;
  * [11..14] iterations that ran anni / comm / call / other redexes,
  * [15..18] ticks inside the rules for each of those classes; [19] redexes
- * popped since the net was reset.  For a single net the tick counts ([2],
- * [7], [8], [15..18]) are 0 unless hvm_gpu_set_sched_timing turned them on;
- * batched and serving launches always count them. */
+ * popped since the net was reset.  For a single net [0..8] and [11..18]
+ * are 0 unless hvm_gpu_set_sched_stats turned them on; batched and serving
+ * launches always collect them. */
 int hvm_gpu_sched_stats(hvm_gpu *h, uint64_t out[20]);
-/* Whether a single net's reduction launches count ticks in the scheduler
- * statistics (off by default: the clock reads and tick sums cost about 2% of
- * its throughput).  No counterpart in the reference. */
-int hvm_gpu_set_sched_timing(hvm_gpu *h, int on);
+/* Whether a single net's reduction launches collect the scheduler
+ * statistics (off by default: keeping them costs 2-4% of its throughput).
+ * No counterpart in the reference. */
+int hvm_gpu_set_sched_stats(hvm_gpu *h, int on);
 
 /* Host-only book utilities (no device).  Text results follow
  * hvm_gpu_readback's buffer convention; errors are described by

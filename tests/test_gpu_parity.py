@@ -600,9 +600,10 @@ def test_large_normal_form_readback(gpu_heap, oracle):
     assert rw == r["rwts"]
 
 
-def test_sched_timing_switch(gpu_heap, oracle):
-    """Scheduler tick statistics only when asked for (hvm_gpu_set_sched_timing);
-    iteration and pop counts either way, and the same result."""
+def test_sched_stats_switch(gpu_heap, oracle):
+    """A single net's scheduler statistics only when asked for
+    (hvm_gpu_set_sched_stats); the pop count either way, and the same
+    result."""
     import hvm64
     from hvm64 import configs
     text = configs.tree_sum(12)
@@ -612,7 +613,7 @@ def test_sched_timing_switch(gpu_heap, oracle):
     out = {}
     try:
         for on in (False, True):
-            gpu_heap.set_sched_timing(on)
+            gpu_heap.set_sched_stats(on)
             net = hvm64.Net(gpu_heap)
             net.boot(host.defs["main"])
             net.normal()
@@ -620,7 +621,7 @@ def test_sched_timing_switch(gpu_heap, oracle):
             assert net.rwts.as_dict() == r["rwts"]
             out[on] = net.sched_stats()
     finally:
-        gpu_heap.set_sched_timing(False)
-    assert out[False]["busy_iters"] > 0 and out[False]["pops"] > 0
-    assert out[False]["busy_ticks"] == 0 and out[False]["call_ticks"] == 0
-    assert out[True]["busy_ticks"] > 0 and out[True]["interact_ticks"] > 0
+        gpu_heap.set_sched_stats(False)
+    assert out[False]["pops"] > 0
+    assert out[False]["busy_iters"] == 0 and out[False]["busy_ticks"] == 0 and out[False]["call_ticks"] == 0
+    assert out[True]["busy_iters"] > 0 and out[True]["busy_ticks"] > 0 and out[True]["interact_ticks"] > 0
