@@ -185,6 +185,13 @@ DEV void fail(Lane &L, u32 code, u32 site = 0) { report_error(L.P->g, code, site
 
 DEV u64 dump_addr() { return HVM_HEAP_OFF + DUMP_WORD * 8; }
 
+// Counters of rare events (interpreter shortcuts, allocation misses) go to
+// the global counters at once: a per-lane register each would be kept for
+// the whole launch.
+DEV void count_rare(Lane &L, u32 k, u32 n) {
+  if (n) HVM_FAA(&L.P->g->rw[k], (u64)n);
+}
+
 // Ring allocation.  Every wave owns a ring (a multiple of 64 nodes) and
 // every lane a 1/64 slice of it (every 64th node, so lanes allocating
 // together write adjacent nodes).  A slice's position counts its allocations since the heap
@@ -403,8 +410,8 @@ DEV u64 alloc_node(Lane &L) {
   L.ov_cur = c.ov_cur;
   L.slice_rest = c.slice_rest;
   L.ov_left = c.ov_left;
-  L.rw[RW_ALLOC_MISS] += c.misses;
-  L.rw[RW_OV_ALLOC] += c.ov_allocs;
+  count_rare(L, RW_ALLOC_MISS, c.misses);
+  count_rare(L, RW_OV_ALLOC, c.ov_allocs);
   if (c.err) fail(L, c.err);
   return c.addr;
 }
@@ -858,7 +865,7 @@ DEV void exec_num(Lane &L, Interp &I, u32 kind, u32 b, u32 c, u64 w1, u64 port, 
       if (div0) fail(L, ERR_DIV0);
       t_set(I, c, numport(out));
     } else if (port == P_ERA) {
-      L.rw[SC_OPNUM_ERA]++;
+      count_rare(L, SC_OPNUM_ERA, 1);
       t_set(I, c, P_ERA);
     } else {
       u64 v, n;
@@ -881,7 +888,7 @@ DEV void exec_num(Lane &L, Interp &I, u32 kind, u32 b, u32 c, u64 w1, u64 port, 
       t_set(I, b, mkport(T_OP, N_SYM, n));
       t_set(I, c, var_port(v));
     } else if (port == P_ERA) {
-      L.rw[SC_OP_ERA]++;
+      count_rare(L, SC_OP_ERA, 1);
       t_set(I, b, P_ERA);
       t_set(I, c, P_ERA);
     } else {
@@ -916,7 +923,7 @@ DEV void exec_num(Lane &L, Interp &I, u32 kind, u32 b, u32 c, u64 w1, u64 port, 
     L.dirty = true;
     t_set(I, c, wire_port(w));
   } else if (port == P_ERA) {
-    L.rw[SC_SWITCH_ERA]++;
+    count_rare(L, SC_SWITCH_ERA, 1);
     L.rw[RW_ERAS]++;
     t_set(I, b, P_ERA);
     t_set(I, c, P_ERA);
@@ -963,7 +970,11 @@ DEV void run_template(Lane &L, Interp &I, u32 off) {
 #undef HVM_TW
   if (h1) {
 #pragma unroll
-    for (u32 k = 0; k < 8; k++) L.rw[k] += (u32)((h1 >> (8 * k)) & 0xFF);
+    for (u32 k = 0; k < 5; k++) L.rw[k] += (u32)((h1 >> (8 * k)) & 0xFF);
+    if (h1 >> 40) {  // the interpreter shortcuts the compiler took (rare)
+#pragma unroll
+      for (u32 k = 5; k < 8; k++) count_rare(L, k, (u32)((h1 >> (8 * k)) & 0xFF));
+    }
   }
 }
 
