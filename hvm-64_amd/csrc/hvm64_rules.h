@@ -816,8 +816,8 @@ DEV bool labs_has(const Lane &L, const DefRec &d, u32 lab) {  // def.rs:30-37
 // the same net with the same rewrite counts (each fast path counts what the
 // interaction it replaces counts).
 struct Interp {
-  const u64 *ip;  // next instruction (global book)
-  u32 ipl;        // next instruction's word in the LDS book
+  u32 ipw;        // next instruction's first word in the instruction array
+                  // (of the LDS copy of the book, from L.book_ins)
   lds_u64 *Tl;    // trg registers 0 .. LDS_SLOTS-1 (LDS, stride 64)
   u64 *Tg;        // trg registers LDS_SLOTS .. (global, stride 64)
   u32 left;       // instructions left; 0: no definition running
@@ -1079,8 +1079,7 @@ DEV void interp_begin(Lane &L, Interp &I, const DefRec &d, u64 t0, u64 cdef) {
       nslots = cdef_nslots(cdef);
     }
   }
-  I.ip = P.instrs + (u64)ioff * 2;
-  I.ipl = L.book_ins + ioff * 2;
+  I.ipw = ioff * 2;
   I.left = ilen;
   if (nslots > P.trg_slots) {  // the trg area is sized for the book
     fail(L, ERR_BADPTR, BAD_SLOTS);
@@ -1099,16 +1098,15 @@ DEV void interp_run(Lane &L, Interp &I, u32 &np) {
   while (I.left > 0 && np < LINK_CAP) {
     u64 w0, w1;
     if (L.book) {
-      w0 = L.book[I.ipl];
-      w1 = L.book[I.ipl + 1];
+      w0 = L.book[L.book_ins + I.ipw];
+      w1 = L.book[L.book_ins + I.ipw + 1];
     } else {
-      w0 = I.ip[0];
-      w1 = I.ip[1];
+      w0 = L.P->instrs[I.ipw];
+      w1 = L.P->instrs[I.ipw + 1];
       HVM_SETTLE_LOADS();
     }
     exec_step(L, I, w0, w1, np);
-    I.ip += 2;
-    I.ipl += 2;
+    I.ipw += 2;
     I.left -= 1;
   }
 }
@@ -1341,8 +1339,7 @@ DEV void task_loop(Lane &L, Interp &I, u32 &np) {
 DEV void run_task(Lane &L, bool has, u64 a, u64 b, u32 di, u64 t0) {
   Interp I;
   I.left = 0;
-  I.ip = nullptr;
-  I.ipl = 0;
+  I.ipw = 0;
   I.Tl = nullptr;
   I.Tg = nullptr;
   I.t_lds = false;
@@ -1378,8 +1375,7 @@ DEV void serve_boot(Lane &L, u32 s, u32 di, u64 tag) {
   L.rw[RW_DREF]++;
   Interp I;
   I.left = 0;
-  I.ip = nullptr;
-  I.ipl = 0;
+  I.ipw = 0;
   I.Tl = nullptr;
   I.Tg = nullptr;
   I.t_lds = false;
