@@ -134,7 +134,7 @@ struct Params {
   u64 *heap;           // heap[(addr - HVM_HEAP_OFF) >> 3] is the word at engine address `addr`
   u64 heap_words;
   u64 ring_node0;      // node index of wave 0's allocation ring
-  u64 wave_nodes;      // nodes per wave ring (a power of two)
+  u64 wave_nodes;      // nodes per wave ring (a multiple of 64)
   u64 ov_node0;        // node index of the shared overflow ring
   u64 ov_nodes;        // nodes in the overflow ring (a power of two)
   u64 *alloc_pos;      // per-wave ring positions, persisted across launches
