@@ -233,7 +233,7 @@ DEV void claim_global(const Params &P, Lane &L, u64 *bag, u32 *top_c, u32 k, u32
     }
     gst(slot, 0);
     gst(slot + 1, 0);
-    const u32 c = redex_class(a, b, L.book, P.ndefs);
+    const u32 c = redex_class(a, b, L.book, has_book(L), P.ndefs);
     const u32 idx = HVM_LDS_FAA(&top_c[c], 1u);
     const u32 i = (ring_base(c) + (idx & (ring_cap(c) - 1))) * 2;
     bag[i] = a;
@@ -338,7 +338,9 @@ DEV void clamp_rings(u32 *top_c, const u32 *bot_c, u32 lane) {
 // TIMING: the launch collects scheduler statistics (iterations, spills,
 // claims, wall-clock ticks): the batched and serving instantiations always;
 // the single-net one when asked for, as keeping them costs it 2-4%.
-template <bool ARENAS, bool SERVE, bool TIMING>
+// LDS: the book and every definition's trg registers fit in LDS (P.book_lds
+// and P.t_lds are set), so the instantiation reads them only from there.
+template <bool ARENAS, bool SERVE, bool TIMING, bool LDS>
 __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   __shared__ u64 s_bag[WAVES_PER_BLOCK][RING_PAIRS * 2];
   __shared__ u64 s_T[WAVES_PER_BLOCK][(LDS_SLOTS + 2 * LINK_CAP) * 64];
@@ -349,7 +351,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   // the book, copied into LDS when it fits (the host decides: P.book_lds)
   const u32 nd = P.ndefs * 4;
   const u32 b_cdef = nd + P.instr_words + P.lab_words, b_tmpl = b_cdef + P.ndefs;
-  if (P.book_lds) {
+  if (LDS || P.book_lds) {
     const u64 *defs = (const u64 *)P.defs;
     for (u32 i = threadIdx.x; i < nd; i += blockDim.x) s_book[i] = defs[i];
     for (u32 i = threadIdx.x; i < P.instr_words; i += blockDim.x) s_book[nd + i] = P.instrs[i];
@@ -372,9 +374,10 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
   lane_init(L, P, gwave, lane);
   L.arenas = ARENAS;
   L.serve = SERVE;
+  L.lds_all = LDS;
   L.T_lds = &s_T[wib][lane];
   L.PL = &s_T[wib][LDS_SLOTS * 64 + lane];
-  if (P.book_lds) {
+  if (LDS || P.book_lds) {
     L.book = HVM_TO_LDS(s_book);
     L.book_ins = nd;
     L.book_lab = nd + P.instr_words;
@@ -487,7 +490,7 @@ __global__ __launch_bounds__(WAVES_PER_BLOCK * 64) void k_reduce(Params P) {
         if (k > 0) {  // a delivery: the sender already counted this wave active
           if (lane < k) {
             const u64 a = gld(my_box + 16 + 2 * lane), b = gld(my_box + 16 + 2 * lane + 1);
-            const u32 c = redex_class(a, b, L.book, P.ndefs);  // the rings are empty: k <= 64 entries fit
+            const u32 c = redex_class(a, b, L.book, has_book(L), P.ndefs);  // the rings are empty: k <= 64 entries fit
             const u32 idx = HVM_LDS_FAA(&top_c[c], 1u);
             const u32 i = (ring_base(c) + (idx & (ring_cap(c) - 1))) * 2;
             bag[i] = a;
@@ -1060,10 +1063,16 @@ hipError_t launch_root_ports(const Params &P, u32 n, const u64 *roots, u64 *out,
 }
 hipError_t launch_reduce(const Params &P, u32 blocks, hipStream_t st) {
   const dim3 grid(blocks), block(WAVES_PER_BLOCK * 64);
-  if (P.serve) hipLaunchKernelGGL((k_reduce<true, true, true>), grid, block, 0, st, P);
-  else if (P.gmap) hipLaunchKernelGGL((k_reduce<true, false, true>), grid, block, 0, st, P);
-  else if (P.timing) hipLaunchKernelGGL((k_reduce<false, false, true>), grid, block, 0, st, P);
-  else hipLaunchKernelGGL((k_reduce<false, false, false>), grid, block, 0, st, P);
+#define HVM_REDUCE(A, S, T)                                                         \
+  do {                                                                           \
+    if (P.book_lds && P.t_lds) hipLaunchKernelGGL((k_reduce<A, S, T, true>), grid, block, 0, st, P); \
+    else hipLaunchKernelGGL((k_reduce<A, S, T, false>), grid, block, 0, st, P);  \
+  } while (0)
+  if (P.serve) HVM_REDUCE(true, true, true);
+  else if (P.gmap) HVM_REDUCE(true, false, true);
+  else if (P.timing) HVM_REDUCE(false, false, true);
+  else HVM_REDUCE(false, false, false);
+#undef HVM_REDUCE
   return hipGetLastError();
 }
 hipError_t launch_readback(const Params &P, u64 root_cell, u64 q_head, u64 q_tail, u64 *stack, u64 stack_cap,
@@ -1076,7 +1085,7 @@ u32 reduce_waves_per_block() { return WAVES_PER_BLOCK; }
 u32 reduce_book_lds_words() { return BOOK_LDS_WORDS; }
 u32 reduce_lds_slots() { return LDS_SLOTS; }
 hipError_t reduce_occupancy(int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_reduce<false, false, false>, WAVES_PER_BLOCK * 64, 0);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_reduce<false, false, false, true>, WAVES_PER_BLOCK * 64, 0);
 }
 
 }  // namespace hvm

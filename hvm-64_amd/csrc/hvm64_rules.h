@@ -149,6 +149,10 @@ struct Lane {
   // (serve: a serving launch, the arenas being request slots; also a
   // compile-time constant in the kernel that serves)
   bool serve;
+  // the book and every definition's trg registers are in LDS: a constant in
+  // the kernel instantiation for such books, which then carries no global
+  // fallback reads nor the per-access branches that choose between the two
+  bool lds_all;
   u32 lane_id;         // the lane's index in its wave
   u32 pushed;          // serving: redexes this task put in the wave's rings
   u32 inl_n;           // serving: redexes without a heap port waiting to run in this task
@@ -182,6 +186,9 @@ struct Lane {
 };
 
 DEV void fail(Lane &L, u32 code, u32 site = 0) { report_error(L.P->g, code, site); }
+
+// The book is read from LDS (L.book) rather than from global memory.
+DEV bool has_book(const Lane &L) { return L.lds_all || L.book != nullptr; }
 
 DEV u64 dump_addr() { return HVM_HEAP_OFF + DUMP_WORD * 8; }
 
@@ -556,14 +563,14 @@ DEV void global_push1(Lane &L, u64 a, u64 b) {
 // instruction count), read from the book in LDS; a wave runs one call ring
 // per iteration and waits for its longest definition, so definitions of
 // similar length share a ring.  Without the LDS book: definition modulo NCALL.
-DEV u32 redex_class(u64 a, u64 b, const lds_u64 *book, u32 ndefs) {
+DEV u32 redex_class(u64 a, u64 b, const lds_u64 *book, bool lds, u32 ndefs) {
   u32 at = ptag(a), bt = ptag(b);
   const bool aref = at == T_REF && a != P_ERA, bref = bt == T_REF && b != P_ERA;
   if (aref || bref) {
     const u64 addr = paddr(aref ? a : b);
     if (addr_in_heap(addr)) return CLS_CALL;
     const u32 di = def_index(addr);
-    const u32 g = book ? (di < ndefs ? (u32)book[4 * di + 3] : 0u) : di;
+    const u32 g = lds ? (di < ndefs ? (u32)book[4 * di + 3] : 0u) : di;
     return CLS_CALL + g % NCALL;
   }
   if (at == bt) {
@@ -595,7 +602,7 @@ DEV void redux(Lane &L, u64 a, u64 b) {
     }
   }
   if (L.bag) {
-    u32 c = redex_class(a, b, L.book, L.P->ndefs);
+    u32 c = redex_class(a, b, L.book, has_book(L), L.P->ndefs);
     u32 idx = HVM_LDS_FAA(&L.bag_top[c], 1u);
     if (idx - L.bag_bot[c] < ring_cap(c)) {
       u32 k = (ring_base(c) + (idx & (ring_cap(c) - 1))) * 2;
@@ -777,7 +784,7 @@ DEV void op_num(Lane &L, u32 &np, u64 a, u64 b) {  // interact.rs:251-270
 // ---------------------------------------------------------------------------
 // A definition record, from the LDS copy of the book when there is one.
 DEV DefRec load_def(const Lane &L, u32 di) {
-  if (L.book) {
+  if (has_book(L)) {
     DefRec d;
     u64 w0 = L.book[4 * di], w1 = L.book[4 * di + 1], w2 = L.book[4 * di + 2];
     d.ioff = (u32)w0;
@@ -797,7 +804,7 @@ DEV bool labs_has(const Lane &L, const DefRec &d, u32 lab) {  // def.rs:30-37
   if (lab >= d.min_safe) return false;
   u32 idx = lab >> 6;
   if (idx >= d.labs_words) return false;
-  u64 w = L.book ? L.book[L.book_lab + d.labs_off + idx] : L.P->labbits[d.labs_off + idx];
+  u64 w = has_book(L) ? L.book[L.book_lab + d.labs_off + idx] : L.P->labbits[d.labs_off + idx];
   return (w >> (lab & 63)) & 1;
 }
 
@@ -958,7 +965,7 @@ DEV u64 tspec(const Interp &I, u64 s) {
 // call began (interp_begin).
 DEV void run_template(Lane &L, Interp &I, u32 off) {
   const Params &P = *L.P;
-  const bool lds = L.book != nullptr;
+  const bool lds = has_book(L);
   const u32 base = lds ? L.book_tmpl + off : off;
 #define HVM_TW(i) (lds ? (u64)L.book[base + (i)] : P.tmpl[base + (i)])
   const u64 h0 = HVM_TW(0), h1 = HVM_TW(1);
@@ -1041,7 +1048,7 @@ DEV void exec_step(Lane &L, Interp &I, u64 w0, u64 w1, u32 &np) {
 // definitions are off).
 DEV u64 load_cdef(const Lane &L, u32 di) {
   if (!L.P->compiled) return 0;
-  return L.book ? (u64)L.book[L.book_cdef + di] : L.P->cdefs[di];
+  return has_book(L) ? (u64)L.book[L.book_cdef + di] : L.P->cdefs[di];
 }
 
 // A compiled definition runs when its template's slots fit, as one block, in
@@ -1053,7 +1060,7 @@ DEV void interp_begin(Lane &L, Interp &I, const DefRec &d, u64 t0, u64 cdef) {
   const Params &P = *L.P;
   I.Tl = HVM_TO_LDS(L.T_lds);
   I.Tg = L.T_glb;
-  I.t_lds = P.t_lds && L.T_lds;
+  I.t_lds = L.lds_all || (P.t_lds && L.T_lds);
   u32 ioff = d.ioff, ilen = d.ilen, nslots = d.nslots;
   I.tbase = 0;
   I.tstride = TSLOT_STRIDE;
@@ -1097,7 +1104,7 @@ DEV void interp_run(Lane &L, Interp &I, u32 &np) {
   HVM_SETTLE_LOADS();
   while (I.left > 0 && np < LINK_CAP) {
     u64 w0, w1;
-    if (L.book) {
+    if (has_book(L)) {
       w0 = L.book[L.book_ins + I.ipw];
       w1 = L.book[L.book_ins + I.ipw + 1];
     } else {
@@ -1342,7 +1349,7 @@ DEV void run_task(Lane &L, bool has, u64 a, u64 b, u32 di, u64 t0) {
   I.ipw = 0;
   I.Tl = nullptr;
   I.Tg = nullptr;
-  I.t_lds = false;
+  I.t_lds = L.lds_all;  // a constant where the kernel knows it
   u32 np = 0;
   if (has && L.arenas) L.cur_net = net_of_port(*L.P, a, net_of_port(*L.P, b, L.cur_net));
   if (L.serve) {
@@ -1378,7 +1385,7 @@ DEV void serve_boot(Lane &L, u32 s, u32 di, u64 tag) {
   I.ipw = 0;
   I.Tl = nullptr;
   I.Tg = nullptr;
-  I.t_lds = false;
+  I.t_lds = L.lds_all;  // a constant where the kernel knows it
   u32 np = 0;
   L.pushed = 0;
   interp_begin(L, I, load_def(L, di), var_port(r), load_cdef(L, di));
@@ -1423,6 +1430,7 @@ DEV void lane_init(Lane &L, const Params &P, u32 gwave, u32 lane) {
   L.bag_top = nullptr;
   L.bag_bot = nullptr;
   L.book = nullptr;
+  L.lds_all = false;
   L.book_ins = 0;
   L.book_lab = 0;
   L.book_cdef = 0;
