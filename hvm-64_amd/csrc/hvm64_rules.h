@@ -946,17 +946,21 @@ DEV void exec_num(Lane &L, Interp &I, u32 kind, u32 b, u32 c, u64 w1, u64 port, 
   }
 }
 
-// A template word spec (hvm64_engine.h) as the word it stands for.
+// A template word spec (hvm64_engine.h) as the word it stands for.  Every
+// candidate is computed and one is selected, with no branch: the lanes of a
+// wave write different templates, and a branch per tag costs each word the
+// exec-mask bookkeeping of every arm.  The register read uses register 0
+// for words that name none (every definition has it: t0).
 DEV u64 tspec(const Interp &I, u64 s) {
   const u32 t = ptag(s);
-  if (t == T_REF || t == T_NUM) return s;
-  if (t >= T_OP) return (s & ~ADDR_MASK) | (I.tbase + (paddr(s) >> 3) * I.tstride);
-  if (t == T_VAR) {
-    const u64 k = paddr(s) >> 3;
-    return var_port(I.tbase + (k >> 1) * I.tstride + (k & 1) * 8);
-  }
-  if (s == P_NONE || s == P_FREE) return s;
-  return norm_endpoint(t_get(I, (u32)(paddr(s) >> 3)));
+  const u64 k = paddr(s) >> 3;
+  const bool var = t == T_VAR;
+  const u64 addr = I.tbase + (var ? k >> 1 : k) * I.tstride + (var ? (k & 1) * 8 : 0);
+  const u64 rel = (s & ~ADDR_MASK) | addr;  // a slot's port; a Var spec's label is 0, so var_port(addr)
+  const bool reg = t == T_RED && s != P_NONE && s != P_FREE;
+  const u64 rv = norm_endpoint(t_get(I, reg ? (u32)k : 0u));
+  const bool lit = t == T_REF || t == T_NUM || (t == T_RED && !reg);
+  return lit ? s : reg ? rv : rel;
 }
 
 // I_TEMPLATE: writes every slot of the running definition's template (one
