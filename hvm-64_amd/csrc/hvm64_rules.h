@@ -989,62 +989,62 @@ DEV void run_template(Lane &L, Interp &I, u32 off) {
   }
 }
 
-// One instruction (instruction.rs:110-210).  The common kinds come first
-// and take short paths: a wave executes the union of its lanes' paths, one
-// instruction every four cycles.
+// One instruction (instruction.rs:110-210).  The kinds that read a trg
+// (Ctr first, the most common) are tested first and the others after: a
+// wave executes the union of its lanes' paths, and each test of a per-lane
+// kind costs every lane the exec-mask bookkeeping of its branch.
 DEV void exec_step(Lane &L, Interp &I, u64 w0, u64 w1, u32 &np) {
   const Params &P = *L.P;
   const u32 kind = (u32)(w0 & 0xFF);
   const u32 a = (u32)(w0 >> 16) & 0xFFFF, b = (u32)(w0 >> 32) & 0xFFFF, c = (u32)(w0 >> 48);
-  if (kind == I_CONST) {
-    t_set(I, a, w1);
+  const u64 t_a = t_get(I, a);  // every kind's `a` is a register of the definition (checked at book load)
+  if (kind - I_CTR <= I_SWITCH - I_CTR) {
+    // Ctr / Op / OpNum / Switch: the trg, followed if it is a wire
+    const u64 ta = (w0 & I_ASWIRE) ? as_wire(t_a) : t_a;
+    const u64 port = ptag(ta) == T_RED ? resolve(L, ta) : ta;
+    if (kind != I_CTR) {
+      exec_num(L, I, kind, b, c, w1, port, np);
+      return;
+    }
+    const u32 lab = (u32)(w1 & 0xFFFF);
+    if (ptag(port) == T_CTR && plab(port) == lab) {  // do_ctr's fast path (instruction.rs:116-121)
+      u64 x, y;
+      take(L, port, x, y);
+      L.rw[RW_ANNI]++;
+      t_set(I, b, as_wire(x));
+      t_set(I, c, as_wire(y));
+      return;
+    }
+    u64 v, n;  // instruction.rs:126-131
+    alloc2(L, v, n);
+    st2_new(P, v, P_NONE, P_NONE);
+    st2_new(P, n, var_port(v), var_port(v + 8));
+    L.dirty = true;
+    push_link(L, np, port, mkport(T_CTR, lab, n));
+    t_set(I, b, var_port(v));
+    t_set(I, c, var_port(v + 8));
     return;
   }
-  if (kind == I_WIRES) {
-    u64 v, unused;
-    v = alloc_node(L);
-    (void)unused;
-    st2_new(P, v, P_NONE, P_NONE);
-    L.dirty = true;
-    t_set(I, a, var_port(v));
-    t_set(I, b, wire_port(v));
-    t_set(I, c, var_port(v + 8));
-    t_set(I, (u32)(w1 & 0xFFFF), wire_port(v + 8));
+  if (kind == I_LINK || kind == I_LINKCONST || kind == I_LINKT) {
+    push_link(L, np, t_a, kind == I_LINK ? t_get(I, b) : kind == I_LINKT ? tspec(I, w1) : w1);
     return;
   }
   if (kind == I_TEMPLATE) {
     run_template(L, I, (u32)w1);
     return;
   }
-  const u64 t_a = t_get(I, a);
-  if (kind == I_LINK || kind == I_LINKCONST || kind == I_LINKT) {
-    push_link(L, np, t_a, kind == I_LINK ? t_get(I, b) : kind == I_LINKT ? tspec(I, w1) : w1);
+  if (kind == I_CONST) {
+    t_set(I, a, w1);
     return;
   }
-  // Ctr / Op / OpNum / Switch: the trg, followed if it is a wire
-  const u64 ta = (w0 & I_ASWIRE) ? as_wire(t_a) : t_a;
-  const u64 port = ptag(ta) == T_RED ? resolve(L, ta) : ta;
-  if (kind != I_CTR) {
-    exec_num(L, I, kind, b, c, w1, port, np);
-    return;
-  }
-  const u32 lab = (u32)(w1 & 0xFFFF);
-  if (ptag(port) == T_CTR && plab(port) == lab) {  // do_ctr's fast path (instruction.rs:116-121)
-    u64 x, y;
-    take(L, port, x, y);
-    L.rw[RW_ANNI]++;
-    t_set(I, b, as_wire(x));
-    t_set(I, c, as_wire(y));
-    return;
-  }
-  u64 v, n;  // instruction.rs:126-131
-  alloc2(L, v, n);
+  // I_WIRES
+  const u64 v = alloc_node(L);
   st2_new(P, v, P_NONE, P_NONE);
-  st2_new(P, n, var_port(v), var_port(v + 8));
   L.dirty = true;
-  push_link(L, np, port, mkport(T_CTR, lab, n));
-  t_set(I, b, var_port(v));
+  t_set(I, a, var_port(v));
+  t_set(I, b, wire_port(v));
   t_set(I, c, var_port(v + 8));
+  t_set(I, (u32)(w1 & 0xFFFF), wire_port(v + 8));
 }
 
 // def.rs:217-289: start the definition whose record is `d`, with t0 = `t0`.
