@@ -1063,10 +1063,12 @@ hipError_t launch_root_ports(const Params &P, u32 n, const u64 *roots, u64 *out,
 }
 hipError_t launch_reduce(const Params &P, u32 blocks, hipStream_t st) {
   const dim3 grid(blocks), block(WAVES_PER_BLOCK * 64);
-#define HVM_REDUCE(A, S, T)                                                         \
-  do {                                                                           \
-    if (P.book_lds && P.t_lds) hipLaunchKernelGGL((k_reduce<A, S, T, true>), grid, block, 0, st, P); \
-    else hipLaunchKernelGGL((k_reduce<A, S, T, false>), grid, block, 0, st, P);  \
+  // the LDS instantiation when the book and every trg register fit there
+  const bool lds = P.book_lds && P.t_lds;
+#define HVM_REDUCE(A, S, T)                                                     \
+  do {                                                                          \
+    if (lds) hipLaunchKernelGGL((k_reduce<A, S, T, true>), grid, block, 0, st, P); \
+    else hipLaunchKernelGGL((k_reduce<A, S, T, false>), grid, block, 0, st, P); \
   } while (0)
   if (P.serve) HVM_REDUCE(true, true, true);
   else if (P.gmap) HVM_REDUCE(true, false, true);
