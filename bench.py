@@ -319,7 +319,7 @@ def batch_roofline(book, table, nets, rewrites, k_ms, args):
     traffic, source = recorded_traffic("batch", args.nets, args.seed)
     out = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
            "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": traffic,
-           "bytes_per_rewrite": round(alg / rewrites, 2), "kernel": "k_reduce<true>"}
+           "bytes_per_rewrite": round(alg / rewrites, 2), "kernel": "k_reduce (per-net arenas)"}
     if source:
         out["traffic_source"] = source
     return out
